@@ -1,0 +1,170 @@
+/*
+ * fwn.h -- C ABI of the MI355X fast winding number engine.
+ *
+ * This is the drop-in boundary under lagrange::winding::FastWindingNumber
+ * (modules/winding/include/lagrange/winding/FastWindingNumber.h:24-108).  The reference
+ * class owns a float/int copy of the mesh and a HDK_Sample::UT_SolidAngle<float,float>
+ * engine (modules/winding/src/FastWindingNumber.cpp:139-196); here the same role is played
+ * by an opaque fwn_engine holding a GPU-built tree in HBM.  All functions are plain C:
+ * pointers and sizes, status codes, no exceptions cross the boundary.
+ *
+ * Entry points and the reference interface each replaces:
+ *   fwn_create            <- FastWindingNumber(const SurfaceMesh&) / Impl::initialize + UT_SolidAngle::init
+ *                            (FastWindingNumber.cpp:142-169, 198-209)
+ *   fwn_destroy           <- ~FastWindingNumber (FastWindingNumber.cpp:212)
+ *   fwn_solid_angle_point <- FastWindingNumber::solid_angle (FastWindingNumber.cpp:180-187, 221-224)
+ *   fwn_is_inside_point   <- FastWindingNumber::is_inside (FastWindingNumber.cpp:171-178, 216-219)
+ *   fwn_solid_angle       <- the per-point calls above issued over an array of points, as
+ *                            volume::mesh_to_volume does per voxel (modules/volume/src/mesh_to_volume.cpp:167-183)
+ *   fwn_create_multi /
+ *   fwn_solid_angle_multi <- no single reference call: many FastWindingNumber objects queried at once
+ *   fwn_export_tree, fwn_query_stats, fwn_bruteforce_f64 -- measurement and verification hooks
+ *                            (no reference counterpart).
+ *
+ * Threading: an engine is immutable after creation.  Query calls on one engine serialise on
+ * an internal mutex (they share the engine's stream and query scratch); calls on different
+ * engines run concurrently.
+ */
+#ifndef FWN_H
+#define FWN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------------------- */
+enum {
+    FWN_OK = 0,
+    FWN_INVALID_ARGUMENT = 1,
+    FWN_OUT_OF_MEMORY = 2,
+    FWN_NO_DEVICE = 3,
+    FWN_HIP_ERROR = 4,
+    FWN_LIMIT_EXCEEDED = 5
+};
+
+/* ---- flags for query calls -------------------------------------------------------------- */
+enum {
+    FWN_DEVICE_POINTERS = 1 /* Q / outputs are device pointers (already resident in HBM) */
+};
+
+typedef struct fwn_engine fwn_engine;
+
+typedef struct fwn_options {
+    float accuracy_scale;     /* beta: far field when |q-p|^2 > beta^2 r^2. Reference default 2 */
+    float near_fp64_rel;      /* FP64 triangle solid angle when q is within this x bbox-diagonal of
+                                 the leaf's bounding sphere (default 1e-4); 0 disables FP64 */
+    int32_t leaf_max;         /* max triangles per leaf bucket (1..8, default 1: every triangle
+                                 carries its own far-field expansion, as the reference's items do) */
+    int32_t fp64_moment_min;  /* tree nodes with at least this many triangles merge moments in FP64 */
+    int32_t device;           /* HIP device ordinal */
+    int32_t waves_per_block;  /* query launch shape (0 = autotuned/default) */
+    int32_t reserved[6];
+} fwn_options;
+
+/* ---- exported tree format (fwn_export_tree) ------------------------------------------------
+ * The engine stores, per mesh, a wide tree of FWN_WIDTH-slot nodes.  Each slot is 32 32-bit
+ * words.  Words 0..26 are float, 27..31 int32:
+ *   0-2  p     area-weighted centroid of the slot's triangles
+ *   3    r2    squared radius bound: every point of the slot's triangles is within sqrt(r2) of p
+ *   4-6  N     sum of area-weighted normals
+ *   7    tr    trace of the first moment  M1_ji = sum_t A_t (c_t-p)_j n_t,i
+ *   8-13 -3 * (M1xx, M1yy, M1zz, M1xy+M1yx, M1yz+M1zy, M1zx+M1xz)
+ *   14-23 7.5 * (S_xxx, S_yyy, S_zzz, 3S_xxy, 3S_xxz, 3S_yyx, 3S_yyz, 3S_zzx, 3S_zzy, 6S_xyz)
+ *        where S is the fully symmetrised second moment  int (x-p)_j (x-p)_k n_i dA
+ *   24-26 -4.5 * t,  t_k = sum_i S_iik
+ *   27   child node index (internal slot) or -1 (leaf slot)
+ *   28   first triangle (position in the engine's sorted triangle array)
+ *   29   triangle count covered by the slot
+ *   30   number of occupied slots of the child node (internal slots), 0 for leaves
+ *   31   reserved (0)
+ * Triangles are stored in tree order, FWN_TRI_WORDS words each: v0, v1, v2 (9 floats),
+ * original facet index (int32), 2 reserved words.
+ */
+#define FWN_WIDTH 8
+#define FWN_SLOT_WORDS 32
+#define FWN_TRI_WORDS 12
+
+enum {
+    FWN_W_PX = 0, FWN_W_PY = 1, FWN_W_PZ = 2, FWN_W_R2 = 3,
+    FWN_W_NX = 4, FWN_W_NY = 5, FWN_W_NZ = 6, FWN_W_TR = 7,
+    FWN_W_A = 8,  /* 6 words */
+    FWN_W_B = 14, /* 10 words */
+    FWN_W_C = 24, /* 3 words */
+    FWN_W_CHILD = 27, FWN_W_FIRST = 28, FWN_W_COUNT = 29, FWN_W_CHILD_NS = 30
+};
+
+typedef struct fwn_tree_info {
+    int64_t num_meshes;
+    int64_t num_nodes;       /* wide nodes over all meshes */
+    int64_t num_triangles;   /* over all meshes */
+    int32_t width;           /* FWN_WIDTH */
+    int32_t max_depth;       /* deepest wide-node level (root = 1) */
+    double build_ms;         /* device time of the last build */
+} fwn_tree_info;
+
+/* per-query work counters written by fwn_query_stats (int32 each) */
+enum { FWN_STAT_TESTS = 0, FWN_STAT_FAR = 1, FWN_STAT_TRI32 = 2, FWN_STAT_TRI64 = 3, FWN_STAT_COUNT = 4 };
+
+/* ---- lifetime ------------------------------------------------------------------------- */
+int fwn_options_default(fwn_options* opt);
+
+/* One mesh.  V: nV x 3 float (row-major), F: nF x 3 int32, host pointers.  The engine copies
+ * what it needs; the caller's arrays may be freed afterwards (FastWindingNumber.cpp:150-168). */
+int fwn_create(fwn_engine** out, const float* V, int64_t nV, const int32_t* F, int64_t nF,
+               const fwn_options* opt);
+
+/* Many meshes in one engine.  Mesh m owns vertices [v_off[m], v_off[m+1]) and facets
+ * [f_off[m], f_off[m+1]) of V / F; facet indices are local to the mesh.  Offsets have
+ * n_meshes+1 entries.  All meshes are built together by the same kernels. */
+int fwn_create_multi(fwn_engine** out, int32_t n_meshes, const float* V, const int64_t* v_off,
+                     const int32_t* F, const int64_t* f_off, const fwn_options* opt);
+
+int fwn_destroy(fwn_engine* e);
+
+/* ---- queries -------------------------------------------------------------------------- */
+/* Solid angle (steradians, as FastWindingNumber::solid_angle) of nQ points Q (nQ x 3 float).
+ * inside_out (nullable) receives (omega / (4 pi) > 0.5) evaluated in double, byte per point
+ * (FastWindingNumber.cpp:177).  flags: FWN_DEVICE_POINTERS. */
+int fwn_solid_angle(fwn_engine* e, const float* Q, int64_t nQ, float* omega_out, uint8_t* inside_out,
+                    int32_t flags);
+
+/* Points of every mesh in one launch: points of mesh m are [q_off[m], q_off[m+1]) of Q. */
+int fwn_solid_angle_multi(fwn_engine* e, const float* Q, const int64_t* q_off, float* omega_out,
+                          uint8_t* inside_out, int32_t flags);
+
+/* Single point, host memory, as the reference's per-point API.  Runs one GPU launch. */
+int fwn_solid_angle_point(fwn_engine* e, const float q[3], float* omega_out);
+int fwn_is_inside_point(fwn_engine* e, const float q[3], uint8_t* inside_out);
+
+/* ---- measurement / verification hooks ---------------------------------------------------- */
+int fwn_tree_info_get(const fwn_engine* e, fwn_tree_info* info);
+
+/* Copy the tree to host memory.  slots: num_nodes*FWN_WIDTH*FWN_SLOT_WORDS words; tris:
+ * num_triangles*FWN_TRI_WORDS words; roots: 2 int32 per mesh (root node, occupied slots);
+ * tau: one float per mesh (absolute FP64 near-field distance).  Any pointer may be NULL. */
+int fwn_export_tree(const fwn_engine* e, uint32_t* slots, uint32_t* tris, int32_t* roots, float* tau);
+
+/* Same traversal as fwn_solid_angle (single mesh: q_off NULL) but also writes FWN_STAT_COUNT
+ * int32 work counters per point (stats_out, host or device per flags). */
+int fwn_query_stats(fwn_engine* e, const float* Q, const int64_t* q_off, int64_t nQ, float* omega_out,
+                    int32_t* stats_out, int32_t flags);
+
+/* FP64 brute force on the GPU: omega_out[i] = sum over all facets of mesh 0 of the exact
+ * triangle solid angle (van Oosterom-Strackee, FP64).  Host pointers unless flags says device. */
+int fwn_bruteforce_f64(fwn_engine* e, const float* Q, int64_t nQ, double* omega_out, int32_t flags);
+
+/* Average device duration (ms) of the traversal kernel in the most recent query call and the
+ * number of launches it took; measured with HIP events on the engine's stream. */
+int fwn_last_timing(const fwn_engine* e, double* traverse_ms, double* total_ms);
+
+/* Thread-local message describing the last failure on this thread. */
+const char* fwn_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FWN_H */

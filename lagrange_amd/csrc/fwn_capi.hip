@@ -1,0 +1,325 @@
+// C ABI of the engine (include/fwn.h).  Every entry point catches internal failures and
+// returns a status code; fwn_last_error() gives the message of the last failure on the
+// calling thread.  Argument checks mirror the reference's runtime asserts
+// (modules/winding/src/FastWindingNumber.cpp:147-148, 202-207), which throw lagrange::Error.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "fwn_internal.h"
+
+namespace fwn {
+
+namespace {
+thread_local std::string g_last_error;
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        FWN_HIP(hipSetDevice(dev));
+    }
+    ~DeviceGuard()
+    {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+template <class Body>
+int guarded(const char* fn, Body&& body)
+{
+    try {
+        body();
+        return FWN_OK;
+    } catch (const Failure& f) {
+        g_last_error = std::string(fn) + ": " + f.msg;
+        return f.status;
+    } catch (const std::bad_alloc&) {
+        g_last_error = std::string(fn) + ": host allocation failed";
+        return FWN_OUT_OF_MEMORY;
+    } catch (...) {
+        g_last_error = std::string(fn) + ": unknown failure";
+        return FWN_HIP_ERROR;
+    }
+}
+
+void check_options(const fwn_options& o)
+{
+    if (!(o.accuracy_scale > 0.0f) || !std::isfinite(o.accuracy_scale))
+        fail(FWN_INVALID_ARGUMENT, "accuracy_scale must be positive and finite");
+    if (!(o.near_fp64_rel >= 0.0f) || !std::isfinite(o.near_fp64_rel))
+        fail(FWN_INVALID_ARGUMENT, "near_fp64_rel must be >= 0");
+    if (o.leaf_max < 1 || o.leaf_max > 8) fail(FWN_INVALID_ARGUMENT, "leaf_max must be in [1, 8]");
+    if (o.fp64_moment_min < 1) fail(FWN_INVALID_ARGUMENT, "fp64_moment_min must be >= 1");
+}
+
+Engine* make_engine(const fwn_options* opt)
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) fail(FWN_NO_DEVICE, "no HIP device available");
+    fwn_options o;
+    fwn_options_default(&o);
+    if (opt) o = *opt;
+    check_options(o);
+    if (o.device < 0 || o.device >= ndev) fail(FWN_NO_DEVICE, "device ordinal out of range");
+    Engine* e = new Engine();
+    e->opt = o;
+    e->device = o.device;
+    return e;
+}
+
+void destroy_engine(Engine* e)
+{
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (auto& ev : e->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+void create_common(fwn_engine** out, int32_t M, const float* V, const int64_t* v_off, const int32_t* F,
+                   const int64_t* f_off, const fwn_options* opt)
+{
+    if (!out) fail(FWN_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (M < 1) fail(FWN_INVALID_ARGUMENT, "at least one mesh is required");
+    if (!v_off || !f_off) fail(FWN_INVALID_ARGUMENT, "offset arrays are NULL");
+    if (v_off[0] != 0 || f_off[0] != 0) fail(FWN_INVALID_ARGUMENT, "offsets must start at 0");
+    for (int m = 0; m < M; ++m)
+        if (v_off[m + 1] < v_off[m] || f_off[m + 1] < f_off[m])
+            fail(FWN_INVALID_ARGUMENT, "offsets must be non-decreasing");
+    if (v_off[M] > 0 && !V) fail(FWN_INVALID_ARGUMENT, "vertex array is NULL");
+    if (f_off[M] > 0 && !F) fail(FWN_INVALID_ARGUMENT, "facet array is NULL");
+    for (int m = 0; m < M; ++m)
+        if (v_off[m + 1] - v_off[m] > (int64_t)INT32_MAX)
+            fail(FWN_LIMIT_EXCEEDED, "a mesh has more than 2^31-1 vertices (indices are int32)");
+    Engine* e = make_engine(opt);
+    try {
+        DeviceGuard g(e->device);
+        FWN_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+        build_engine(*e, M, V, v_off, F, f_off);
+    } catch (...) {
+        destroy_engine(e);
+        throw;
+    }
+    *out = reinterpret_cast<fwn_engine*>(e);
+}
+
+Engine* E(fwn_engine* h)
+{
+    if (!h) fail(FWN_INVALID_ARGUMENT, "engine is NULL");
+    return reinterpret_cast<Engine*>(h);
+}
+const Engine* E(const fwn_engine* h)
+{
+    if (!h) fail(FWN_INVALID_ARGUMENT, "engine is NULL");
+    return reinterpret_cast<const Engine*>(h);
+}
+} // namespace
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+void fail(int status, const std::string& msg) { throw Failure{status, msg}; }
+
+void hip_check(hipError_t e, const char* what, const char* file, int line)
+{
+    if (e == hipSuccess) return;
+    int st = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? FWN_OUT_OF_MEMORY : FWN_HIP_ERROR;
+    fail(st, std::string(what) + " failed: " + hipGetErrorString(e) + " (" + file + ":" + std::to_string(line) +
+                 ")");
+}
+
+void DBuf::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+void DBuf::alloc(size_t n)
+{
+    release();
+    if (n == 0) n = 16;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+        p = nullptr;
+        (void)hipGetLastError();
+        fail(FWN_OUT_OF_MEMORY, "device allocation of " + std::to_string(n) + " bytes failed");
+    }
+    bytes = n;
+}
+
+void DBuf::reserve(size_t n)
+{
+    if (n <= bytes && p) return;
+    alloc(n + n / 4);
+}
+
+} // namespace fwn
+
+using fwn::E;
+using fwn::guarded;
+
+#pragma GCC visibility push(default)
+extern "C" {
+
+int fwn_options_default(fwn_options* opt)
+{
+    if (!opt) {
+        fwn::g_last_error = "fwn_options_default: opt is NULL";
+        return FWN_INVALID_ARGUMENT;
+    }
+    std::memset(opt, 0, sizeof(*opt));
+    opt->accuracy_scale = 2.0f;
+    opt->near_fp64_rel = 1e-4f;
+    opt->leaf_max = 1;
+    opt->fp64_moment_min = 256;
+    opt->device = 0;
+    opt->waves_per_block = 0;
+    return FWN_OK;
+}
+
+int fwn_create(fwn_engine** out, const float* V, int64_t nV, const int32_t* F, int64_t nF, const fwn_options* opt)
+{
+    return guarded("fwn_create", [&] {
+        if (nV < 0 || nF < 0) fwn::fail(FWN_INVALID_ARGUMENT, "negative size");
+        int64_t vo[2] = {0, nV}, fo[2] = {0, nF};
+        fwn::create_common(out, 1, V, vo, F, fo, opt);
+    });
+}
+
+int fwn_create_multi(fwn_engine** out, int32_t n_meshes, const float* V, const int64_t* v_off, const int32_t* F,
+                     const int64_t* f_off, const fwn_options* opt)
+{
+    return guarded("fwn_create_multi", [&] { fwn::create_common(out, n_meshes, V, v_off, F, f_off, opt); });
+}
+
+int fwn_destroy(fwn_engine* h)
+{
+    return guarded("fwn_destroy", [&] { fwn::destroy_engine(reinterpret_cast<fwn::Engine*>(h)); });
+}
+
+int fwn_solid_angle(fwn_engine* h, const float* Q, int64_t nQ, float* omega_out, uint8_t* inside_out, int32_t flags)
+{
+    return guarded("fwn_solid_angle", [&] {
+        fwn::Engine* e = E(h);
+        if (nQ < 0) fwn::fail(FWN_INVALID_ARGUMENT, "negative point count");
+        if (nQ > 0 && (!Q || !omega_out)) fwn::fail(FWN_INVALID_ARGUMENT, "NULL points or output");
+        if (e->n_meshes != 1) fwn::fail(FWN_INVALID_ARGUMENT, "multi-mesh engine: use fwn_solid_angle_multi");
+        std::lock_guard<std::mutex> lk(e->mu);
+        fwn::DeviceGuard g(e->device);
+        fwn::run_queries(*e, Q, nullptr, nQ, omega_out, inside_out, nullptr, (flags & FWN_DEVICE_POINTERS) != 0);
+    });
+}
+
+int fwn_solid_angle_multi(fwn_engine* h, const float* Q, const int64_t* q_off, float* omega_out, uint8_t* inside_out,
+                          int32_t flags)
+{
+    return guarded("fwn_solid_angle_multi", [&] {
+        fwn::Engine* e = E(h);
+        if (!q_off) fwn::fail(FWN_INVALID_ARGUMENT, "q_off is NULL");
+        if (q_off[e->n_meshes] > 0 && (!Q || !omega_out)) fwn::fail(FWN_INVALID_ARGUMENT, "NULL points or output");
+        std::lock_guard<std::mutex> lk(e->mu);
+        fwn::DeviceGuard g(e->device);
+        fwn::run_queries(*e, Q, q_off, 0, omega_out, inside_out, nullptr, (flags & FWN_DEVICE_POINTERS) != 0);
+    });
+}
+
+int fwn_solid_angle_point(fwn_engine* h, const float q[3], float* omega_out)
+{
+    return guarded("fwn_solid_angle_point", [&] {
+        fwn::Engine* e = E(h);
+        if (!q || !omega_out) fwn::fail(FWN_INVALID_ARGUMENT, "NULL point or output");
+        if (e->n_meshes != 1) fwn::fail(FWN_INVALID_ARGUMENT, "multi-mesh engine has no single-point query");
+        std::lock_guard<std::mutex> lk(e->mu);
+        fwn::DeviceGuard g(e->device);
+        fwn::run_queries(*e, q, nullptr, 1, omega_out, nullptr, nullptr, false);
+    });
+}
+
+int fwn_is_inside_point(fwn_engine* h, const float q[3], uint8_t* inside_out)
+{
+    return guarded("fwn_is_inside_point", [&] {
+        fwn::Engine* e = E(h);
+        if (!q || !inside_out) fwn::fail(FWN_INVALID_ARGUMENT, "NULL point or output");
+        if (e->n_meshes != 1) fwn::fail(FWN_INVALID_ARGUMENT, "multi-mesh engine has no single-point query");
+        std::lock_guard<std::mutex> lk(e->mu);
+        fwn::DeviceGuard g(e->device);
+        float om = 0.0f;
+        fwn::run_queries(*e, q, nullptr, 1, &om, inside_out, nullptr, false);
+    });
+}
+
+int fwn_tree_info_get(const fwn_engine* h, fwn_tree_info* info)
+{
+    return guarded("fwn_tree_info_get", [&] {
+        const fwn::Engine* e = E(h);
+        if (!info) fwn::fail(FWN_INVALID_ARGUMENT, "info is NULL");
+        info->num_meshes = e->n_meshes;
+        info->num_nodes = e->n_nodes;
+        info->num_triangles = e->n_tris;
+        info->width = fwn::kWidth;
+        info->max_depth = e->max_depth;
+        info->build_ms = e->build_ms;
+    });
+}
+
+int fwn_export_tree(const fwn_engine* h, uint32_t* slots, uint32_t* tris, int32_t* roots, float* tau)
+{
+    return guarded("fwn_export_tree", [&] {
+        fwn::Engine* e = const_cast<fwn::Engine*>(E(h));
+        std::lock_guard<std::mutex> lk(e->mu);
+        fwn::DeviceGuard g(e->device);
+        if (slots && e->n_nodes)
+            FWN_HIP(hipMemcpy(slots, e->slots.p, sizeof(uint32_t) * fwn::kWidth * fwn::kSlotWords * e->n_nodes,
+                              hipMemcpyDeviceToHost));
+        if (tris && e->n_tris)
+            FWN_HIP(hipMemcpy(tris, e->tris.p, sizeof(uint32_t) * fwn::kTriWords * e->n_tris, hipMemcpyDeviceToHost));
+        if (roots) std::memcpy(roots, e->h_roots.data(), sizeof(int32_t) * e->h_roots.size());
+        if (tau) std::memcpy(tau, e->h_tau.data(), sizeof(float) * e->h_tau.size());
+    });
+}
+
+int fwn_query_stats(fwn_engine* h, const float* Q, const int64_t* q_off, int64_t nQ, float* omega_out,
+                    int32_t* stats_out, int32_t flags)
+{
+    return guarded("fwn_query_stats", [&] {
+        fwn::Engine* e = E(h);
+        if (!stats_out || !omega_out) fwn::fail(FWN_INVALID_ARGUMENT, "NULL output");
+        std::lock_guard<std::mutex> lk(e->mu);
+        fwn::DeviceGuard g(e->device);
+        fwn::run_queries(*e, Q, q_off, nQ, omega_out, nullptr, stats_out, (flags & FWN_DEVICE_POINTERS) != 0);
+    });
+}
+
+int fwn_bruteforce_f64(fwn_engine* h, const float* Q, int64_t nQ, double* omega_out, int32_t flags)
+{
+    return guarded("fwn_bruteforce_f64", [&] {
+        fwn::Engine* e = E(h);
+        if (nQ < 0) fwn::fail(FWN_INVALID_ARGUMENT, "negative point count");
+        if (nQ > 0 && (!Q || !omega_out)) fwn::fail(FWN_INVALID_ARGUMENT, "NULL points or output");
+        std::lock_guard<std::mutex> lk(e->mu);
+        fwn::DeviceGuard g(e->device);
+        fwn::run_bruteforce(*e, Q, nQ, omega_out, (flags & FWN_DEVICE_POINTERS) != 0);
+    });
+}
+
+int fwn_last_timing(const fwn_engine* h, double* traverse_ms, double* total_ms)
+{
+    return guarded("fwn_last_timing", [&] {
+        const fwn::Engine* e = E(h);
+        if (traverse_ms) *traverse_ms = e->last_traverse_ms;
+        if (total_ms) *total_ms = e->last_total_ms;
+    });
+}
+
+const char* fwn_last_error(void) { return fwn::g_last_error.c_str(); }
+
+} // extern "C"
+#pragma GCC visibility pop

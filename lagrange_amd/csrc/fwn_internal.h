@@ -1,0 +1,169 @@
+// Internal declarations of the MI355X fast winding number engine (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fwn.h"
+
+namespace fwn {
+
+// ---------------------------------------------------------------------------------------
+// error plumbing: C-ABI functions never throw; internal code throws fwn::Failure which the
+// ABI layer turns into a status code + thread-local message.
+// ---------------------------------------------------------------------------------------
+struct Failure {
+    int status;
+    std::string msg;
+};
+
+void set_error(const std::string& msg);
+[[noreturn]] void fail(int status, const std::string& msg);
+void hip_check(hipError_t e, const char* what, const char* file, int line);
+
+#define FWN_HIP(call) ::fwn::hip_check((call), #call, __FILE__, __LINE__)
+#define FWN_LAUNCH_CHECK(name) ::fwn::hip_check(hipGetLastError(), name, __FILE__, __LINE__)
+
+// ---------------------------------------------------------------------------------------
+// constants shared by host and device code
+// ---------------------------------------------------------------------------------------
+constexpr int kWidth = FWN_WIDTH;          // slots per wide node
+constexpr int kSlotWords = FWN_SLOT_WORDS; // 32-bit words per slot
+constexpr int kTriWords = FWN_TRI_WORDS;   // 32-bit words per stored triangle
+constexpr int kRecWords = 32;              // doubles per binary-node moment record
+constexpr int kStack = 192;                // per-wave traversal stack entries
+constexpr int kWave = 64;
+
+// binary-node moment record layout (doubles)
+enum : int { R_A = 0, R_P = 1, R_N = 4, R_M1 = 7, R_S = 16, R_LO = 26, R_HI = 29 };
+
+// RAII device buffer
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release();
+    void alloc(size_t n);          // exact (re)allocation, contents undefined
+    void reserve(size_t n);        // grow only
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct Engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    fwn_options opt{};
+
+    int32_t n_meshes = 0;
+    int64_t n_tris = 0;
+    int64_t n_nodes = 0;
+    int32_t max_depth = 0;
+    double build_ms = 0.0;
+
+    // tree (resident)
+    DBuf slots;      // n_nodes * kWidth * kSlotWords u32
+    DBuf tris;       // n_tris * kTriWords u32
+    DBuf roots;      // 2 int32 per mesh: wide root, occupied slots
+    DBuf mesh_info;  // per mesh: 8 floats: qbox lo[3], qbox inv[3], tau, pad
+    std::vector<int32_t> h_roots;
+    std::vector<float> h_tau;
+    std::vector<int64_t> h_f_off;
+
+    // query scratch (grown on demand)
+    DBuf q_keys[2], q_vals[2], q_hist, q_rowsum, q_in, q_om, q_in8, q_off, q_boff, q_stats, q_flag;
+    DBuf q_bf;
+
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double last_traverse_ms = 0.0;
+    double last_total_ms = 0.0;
+};
+
+// sorting / scanning primitives (fwn_sort.hip)
+struct SortScratch {
+    DBuf* keys2;
+    DBuf* vals2;
+    DBuf* hist;
+    DBuf* rowsum;
+};
+// Stable LSD radix sort of (keys, vals) pairs by key bits [0, nbits).  Result ends in keys/vals
+// (swapping with the scratch buffers as needed; the pointers are updated).
+void radix_sort_pairs(uint64_t*& keys, uint32_t*& vals, uint64_t*& keys_alt, uint32_t*& vals_alt, int64_t n,
+                      int nbits, DBuf& hist, DBuf& rowsum, hipStream_t s);
+// exclusive scan of int32 (in may alias out); returns nothing, total in out[n] if out has n+1 slots
+void exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, DBuf& tmp, hipStream_t s);
+
+// build (fwn_build.hip)
+void build_engine(Engine& e, int32_t n_meshes, const float* V_host, const int64_t* v_off, const int32_t* F_host,
+                  const int64_t* f_off);
+
+// queries (fwn_query.hip)
+void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t nQ, float* omega, uint8_t* inside,
+                 int32_t* stats, bool device_ptrs);
+void run_bruteforce(Engine& e, const float* Q, int64_t nQ, double* out, bool device_ptrs);
+
+int grid_blocks(int64_t n, int threads);
+
+} // namespace fwn
+
+// ---------------------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------------------
+namespace fwn {
+
+__device__ __forceinline__ uint32_t expand10(uint32_t v)
+{
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// (c - lo) * inv scaled to [0, 1023]; same float operations as the CPU check
+__device__ __forceinline__ uint32_t quant10(float c, float lo, float inv)
+{
+    float t = (c - lo) * inv;
+    float s = t * 1024.0f;
+    if (!(s > 0.0f)) return 0u;
+    if (s >= 1023.0f) return 1023u;
+    return (uint32_t)s;
+}
+
+__device__ __forceinline__ uint32_t morton30(float x, float y, float z, const float* lo, const float* inv)
+{
+    return (expand10(quant10(x, lo[0], inv[0])) << 2) | (expand10(quant10(y, lo[1], inv[1])) << 1) |
+           expand10(quant10(z, lo[2], inv[2]));
+}
+
+// order-preserving float <-> int maps for atomic min/max
+__device__ __forceinline__ int32_t f2ord(float f)
+{
+    int32_t i = __float_as_int(f);
+    return i >= 0 ? i : (i ^ 0x7FFFFFFF);
+}
+__host__ __device__ inline float ord2f(int32_t i)
+{
+    int32_t j = i >= 0 ? i : (i ^ 0x7FFFFFFF);
+    float f;
+    __builtin_memcpy(&f, &j, 4);
+    return f;
+}
+
+// XCD-aware block remap: hardware hands consecutive workgroups to the 8 XCDs round-robin;
+// this bijection gives each XCD one contiguous run of logical blocks (shared L2 reuse).
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb)
+{
+    uint32_t per = nb / 8u, rem = nb % 8u;
+    uint32_t x = b % 8u, i = b / 8u;
+    return (x < rem) ? x * (per + 1u) + i : rem * (per + 1u) + (x - rem) * per + i;
+}
+
+} // namespace fwn

@@ -1,0 +1,403 @@
+// Winding number queries on gfx950.
+//
+// Replaces UT_SolidAngle<float,float>::computeSolidAngle, called per point by the reference at
+// modules/winding/src/FastWindingNumber.cpp:177 (is_inside) and :186 (solid_angle), and the
+// per-voxel loop of modules/volume/src/mesh_to_volume.cpp:167-183 that issues those calls.
+//
+// Points are sorted by mesh and 30-bit Morton code, then each wave64 takes 64 consecutive
+// points (one per lane) and walks its mesh's wide tree once for all of them:
+//  * the wave pops a (node, lane mask) pair from a wave-private stack held in three VGPRs
+//    (entry i lives in lane i%64 of bank i/64; pushes are v_cndmask, pops v_readlane), so
+//    node ids and masks stay scalar and slot/triangle loads are scalar (s_load) broadcasts;
+//  * for every slot each active lane tests |q - p|^2 > beta^2 r2: far lanes add the order-2
+//    expansion, near lanes either take the exact triangle solid angles (leaf slot) or stay
+//    active in the child pushed with their mask;
+//  * a lane's own sequence of tests, far terms and triangles is exactly the single-point
+//    traversal (children pushed in slot order, popped last-in-first-out), so each point gets
+//    the same result the per-point reference loop would give on this tree.
+// Triangles closer than tau (near_fp64_rel x bbox diagonal) to the leaf's bounding sphere
+// use the FP64 van Oosterom-Strackee formula.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "fwn_internal.h"
+
+namespace fwn {
+
+namespace {
+
+constexpr int QT = 256; // key kernel threads
+
+__device__ __forceinline__ int mesh_of64(const int64_t* off, int M, int64_t t)
+{
+    int lo = 0, hi = M - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(QT) void k_query_keys(const float* __restrict__ Q, int64_t nQ,
+                                                   const int64_t* __restrict__ q_off, int M,
+                                                   const float* __restrict__ minfo, uint64_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ vals)
+{
+    int64_t i = (int64_t)blockIdx.x * QT + threadIdx.x;
+    if (i >= nQ) return;
+    int m = (M > 1) ? mesh_of64(q_off, M, i) : 0;
+    const float* mi = minfo + 8 * m;
+    uint32_t mc = morton30(Q[3 * i], Q[3 * i + 1], Q[3 * i + 2], mi, mi + 3);
+    keys[i] = ((uint64_t)(uint32_t)m << 32) | mc;
+    vals[i] = (uint32_t)i;
+}
+
+// exact triangle solid angle, FP32, normalised-vector van Oosterom-Strackee form
+__device__ __forceinline__ float tri_f32(const float* __restrict__ t, float qx, float qy, float qz)
+{
+    float ax = t[0] - qx, ay = t[1] - qy, az = t[2] - qz;
+    float bx = t[3] - qx, by = t[4] - qy, bz = t[5] - qz;
+    float cx = t[6] - qx, cy = t[7] - qy, cz = t[8] - qz;
+    float la2 = fmaf(ax, ax, fmaf(ay, ay, az * az));
+    float lb2 = fmaf(bx, bx, fmaf(by, by, bz * bz));
+    float lc2 = fmaf(cx, cx, fmaf(cy, cy, cz * cz));
+    if (la2 == 0.0f || lb2 == 0.0f || lc2 == 0.0f) return 0.0f;
+    float ia = __builtin_amdgcn_rsqf(la2), ib = __builtin_amdgcn_rsqf(lb2), ic = __builtin_amdgcn_rsqf(lc2);
+    ax *= ia; ay *= ia; az *= ia;
+    bx *= ib; by *= ib; bz *= ib;
+    cx *= ic; cy *= ic; cz *= ic;
+    float ux = bx - ax, uy = by - ay, uz = bz - az;
+    float vx = cx - ax, vy = cy - ay, vz = cz - az;
+    float nx = fmaf(uy, vz, -uz * vy);
+    float ny = fmaf(uz, vx, -ux * vz);
+    float nz = fmaf(ux, vy, -uy * vx);
+    float num = fmaf(ax, nx, fmaf(ay, ny, az * nz));
+    if (num == 0.0f) return 0.0f;
+    float ab = fmaf(ax, bx, fmaf(ay, by, az * bz));
+    float ac = fmaf(ax, cx, fmaf(ay, cy, az * cz));
+    float bc = fmaf(bx, cx, fmaf(by, cy, bz * cz));
+    float den = 1.0f + ab + ac + bc;
+    return 2.0f * atan2f(num, den);
+}
+
+// exact triangle solid angle, FP64, 2 atan2(a.(b x c), |a||b||c| + (a.b)|c| + (b.c)|a| + (c.a)|b|)
+__device__ __forceinline__ double tri_f64(const float* __restrict__ t, float qx, float qy, float qz)
+{
+    double q0 = qx, q1 = qy, q2 = qz;
+    double ax = (double)t[0] - q0, ay = (double)t[1] - q1, az = (double)t[2] - q2;
+    double bx = (double)t[3] - q0, by = (double)t[4] - q1, bz = (double)t[5] - q2;
+    double cx = (double)t[6] - q0, cy = (double)t[7] - q1, cz = (double)t[8] - q2;
+    double la = sqrt(ax * ax + ay * ay + az * az);
+    double lb = sqrt(bx * bx + by * by + bz * bz);
+    double lc = sqrt(cx * cx + cy * cy + cz * cz);
+    if (la == 0.0 || lb == 0.0 || lc == 0.0) return 0.0;
+    double num = ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx);
+    if (num == 0.0) return 0.0;
+    double den = la * lb * lc + (ax * bx + ay * by + az * bz) * lc + (bx * cx + by * cy + bz * cz) * la +
+                 (cx * ax + cy * ay + cz * az) * lb;
+    return 2.0 * atan2(num, den);
+}
+
+// order-2 far field of one slot at offset d = p - q (fwn.h slot words)
+__device__ __forceinline__ float far_field(const float* __restrict__ w, float dx, float dy, float dz, float d2)
+{
+    float inv = __builtin_amdgcn_rsqf(d2);
+    float inv2 = inv * inv;
+    float inv3 = inv2 * inv;
+    float inv5 = inv3 * inv2;
+    float inv7 = inv5 * inv2;
+    float o0 = fmaf(dx, w[FWN_W_NX], fmaf(dy, w[FWN_W_NY], dz * w[FWN_W_NZ]));
+    float xx = dx * dx, yy = dy * dy, zz = dz * dz;
+    float xy = dx * dy, yz = dy * dz, zx = dz * dx;
+    const float* A = w + FWN_W_A;
+    float q1 = fmaf(A[0], xx, fmaf(A[1], yy, fmaf(A[2], zz, fmaf(A[3], xy, fmaf(A[4], yz, A[5] * zx)))));
+    q1 = fmaf(w[FWN_W_C], dx, fmaf(w[FWN_W_C + 1], dy, fmaf(w[FWN_W_C + 2], dz, q1)));
+    const float* B = w + FWN_W_B;
+    float q2 = fmaf(B[0], xx * dx, fmaf(B[1], yy * dy, fmaf(B[2], zz * dz,
+               fmaf(B[3], xx * dy, fmaf(B[4], xx * dz, fmaf(B[5], yy * dx,
+               fmaf(B[6], yy * dz, fmaf(B[7], zz * dx, fmaf(B[8], zz * dy, B[9] * (xy * dz))))))))));
+    return fmaf(o0 + w[FWN_W_TR], inv3, fmaf(q1, inv5, q2 * inv7));
+}
+
+// ---- wave stack in VGPRs: entry i sits in lane (i & 63) of bank (i >> 6) -----------------
+// Named registers (no arrays) so the banks stay in VGPRs rather than private memory.
+struct WaveStack {
+    uint32_t n0 = 0, n1 = 0, n2 = 0, l0 = 0, l1 = 0, l2 = 0, h0 = 0, h1 = 0, h2 = 0;
+};
+
+// Branch-free: every bank is updated / read under uniform predicates, so nothing invites the
+// optimiser to turn the banks into an indexed private array.
+__device__ __forceinline__ void stack_push(WaveStack& S, int sp, int lane, uint32_t e, uint64_t mk)
+{
+    const bool me = lane == (sp & 63);
+    const bool b0 = me && sp < 64, b1 = me && sp >= 64 && sp < 128, b2 = me && sp >= 128;
+    const uint32_t mlo = (uint32_t)mk, mhi = (uint32_t)(mk >> 32);
+    S.n0 = b0 ? e : S.n0;
+    S.l0 = b0 ? mlo : S.l0;
+    S.h0 = b0 ? mhi : S.h0;
+    S.n1 = b1 ? e : S.n1;
+    S.l1 = b1 ? mlo : S.l1;
+    S.h1 = b1 ? mhi : S.h1;
+    S.n2 = b2 ? e : S.n2;
+    S.l2 = b2 ? mlo : S.l2;
+    S.h2 = b2 ? mhi : S.h2;
+}
+
+__device__ __forceinline__ void stack_pop(const WaveStack& S, int sp, uint32_t& e, uint64_t& mk)
+{
+    const int l = sp & 63;
+    const uint32_t a0 = __builtin_amdgcn_readlane(S.n0, l), a1 = __builtin_amdgcn_readlane(S.n1, l),
+                   a2 = __builtin_amdgcn_readlane(S.n2, l);
+    const uint32_t c0 = __builtin_amdgcn_readlane(S.l0, l), c1 = __builtin_amdgcn_readlane(S.l1, l),
+                   c2 = __builtin_amdgcn_readlane(S.l2, l);
+    const uint32_t d0 = __builtin_amdgcn_readlane(S.h0, l), d1 = __builtin_amdgcn_readlane(S.h1, l),
+                   d2 = __builtin_amdgcn_readlane(S.h2, l);
+    const uint32_t lo = sp < 64 ? c0 : (sp < 128 ? c1 : c2);
+    const uint32_t hi = sp < 64 ? d0 : (sp < 128 ? d1 : d2);
+    e = sp < 64 ? a0 : (sp < 128 ? a1 : a2);
+    mk = ((uint64_t)hi << 32) | lo;
+}
+
+template <bool STATS, int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void k_traverse(
+    const uint32_t* __restrict__ slots, const uint32_t* __restrict__ tris, const int32_t* __restrict__ roots,
+    const float* __restrict__ minfo, const float* __restrict__ Q, const uint32_t* __restrict__ perm,
+    const int64_t* __restrict__ q_off, const int64_t* __restrict__ b_off, int M, int64_t n_batches, float beta2,
+    float* __restrict__ omega, uint8_t* __restrict__ inside, int32_t* __restrict__ stats,
+    int32_t* __restrict__ flag)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t batch = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * WAVES + wave;
+    if (batch >= n_batches) return; // whole wave
+    const int m = mesh_of64(b_off, M, batch);
+    const int64_t start = q_off[m] + (batch - b_off[m]) * 64;
+    const int64_t cnt64 = q_off[m + 1] - start;
+    const int cnt = cnt64 < 64 ? (int)cnt64 : 64;
+    const bool valid = lane < cnt;
+    const uint32_t orig = valid ? (perm ? perm[start + lane] : (uint32_t)(start + lane)) : 0u;
+    const float qx = valid ? Q[3 * (int64_t)orig] : 0.0f;
+    const float qy = valid ? Q[3 * (int64_t)orig + 1] : 0.0f;
+    const float qz = valid ? Q[3 * (int64_t)orig + 2] : 0.0f;
+    const int32_t root = roots[2 * m];
+    const int32_t root_ns = roots[2 * m + 1];
+    const float tau = minfo[8 * m + 6];
+
+    float om = 0.0f;
+    int32_t n_test = 0, n_far = 0, n_t32 = 0, n_t64 = 0;
+    if (root >= 0) {
+        WaveStack S;
+        stack_push(S, 0, lane, ((uint32_t)root << 4) | (uint32_t)root_ns, __ballot(valid));
+        int sp = 1;
+        bool overflow = false;
+        while (sp > 0) {
+            --sp;
+            uint32_t e;
+            uint64_t mk;
+            stack_pop(S, sp, e, mk);
+            const bool act = (mk >> lane) & 1ull;
+            const uint32_t* nb = slots + (size_t)(e >> 4) * (kWidth * kSlotWords);
+            const int ns = (int)(e & 15u);
+            for (int s = 0; s < ns; ++s) {
+                const float* w = reinterpret_cast<const float*>(nb + s * kSlotWords);
+                const float dx = w[FWN_W_PX] - qx, dy = w[FWN_W_PY] - qy, dz = w[FWN_W_PZ] - qz;
+                const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                const float r2 = w[FWN_W_R2];
+                const bool far = d2 > beta2 * r2;
+                if (STATS) n_test += act ? 1 : 0;
+                if (act && far) {
+                    om += far_field(w, dx, dy, dz, d2);
+                    if (STATS) ++n_far;
+                }
+                const uint64_t nearm = __ballot(act && !far);
+                if (nearm == 0ull) continue;
+                const int32_t child = __float_as_int(w[FWN_W_CHILD]);
+                if (child < 0) {
+                    const int32_t first = __float_as_int(w[FWN_W_FIRST]);
+                    const int32_t tcnt = __float_as_int(w[FWN_W_COUNT]);
+                    const bool nl = (nearm >> lane) & 1ull;
+                    const float rr = sqrtf(r2) + tau;
+                    const bool use64 = nl && tau > 0.0f && d2 < rr * rr;
+                    for (int32_t t = first; t < first + tcnt; ++t) {
+                        const float* tw = reinterpret_cast<const float*>(tris + (size_t)t * kTriWords);
+                        if (nl) {
+                            if (use64) {
+                                om += (float)tri_f64(tw, qx, qy, qz);
+                                if (STATS) ++n_t64;
+                            } else {
+                                om += tri_f32(tw, qx, qy, qz);
+                                if (STATS) ++n_t32;
+                            }
+                        }
+                    }
+                } else if (sp < kStack) {
+                    const uint32_t cns = (uint32_t)__float_as_int(w[FWN_W_CHILD_NS]);
+                    stack_push(S, sp, lane, ((uint32_t)child << 4) | cns, nearm);
+                    ++sp;
+                } else {
+                    overflow = true;
+                }
+            }
+        }
+        if (overflow && lane == 0) atomicOr(flag, 1);
+    }
+    if (valid) {
+        omega[orig] = om;
+        if (inside) inside[orig] = ((double)om / (4.0 * 3.14159265358979323846) > 0.5) ? 1 : 0;
+        if (STATS) {
+            stats[4 * (int64_t)orig + 0] = n_test;
+            stats[4 * (int64_t)orig + 1] = n_far;
+            stats[4 * (int64_t)orig + 2] = n_t32;
+            stats[4 * (int64_t)orig + 3] = n_t64;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bruteforce(const uint32_t* __restrict__ tris, int64_t first, int64_t cnt,
+                                                    const float* __restrict__ Q, int64_t nQ,
+                                                    double* __restrict__ out)
+{
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    bool valid = i < nQ;
+    float qx = valid ? Q[3 * i] : 0.0f, qy = valid ? Q[3 * i + 1] : 0.0f, qz = valid ? Q[3 * i + 2] : 0.0f;
+    double s = 0.0;
+    for (int64_t t = first; t < first + cnt; ++t)
+        s += tri_f64(reinterpret_cast<const float*>(tris + t * kTriWords), qx, qy, qz);
+    if (valid) out[i] = s;
+}
+
+__global__ void k_zero_i32(int32_t* p) { *p = 0; }
+
+} // namespace
+
+void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t nQ, float* omega, uint8_t* inside,
+                 int32_t* stats, bool dev)
+{
+    hipStream_t s = e.stream;
+    const int M = e.n_meshes;
+    std::vector<int64_t> qoff(M + 1);
+    if (q_off_host) {
+        for (int m = 0; m <= M; ++m) qoff[m] = q_off_host[m];
+        if (qoff[0] != 0) fail(FWN_INVALID_ARGUMENT, "q_off[0] must be 0");
+        for (int m = 0; m < M; ++m)
+            if (qoff[m + 1] < qoff[m]) fail(FWN_INVALID_ARGUMENT, "q_off must be non-decreasing");
+        nQ = qoff[M];
+    } else {
+        if (M != 1) fail(FWN_INVALID_ARGUMENT, "per-mesh point offsets are required for a multi-mesh engine");
+        qoff[0] = 0;
+        qoff[1] = nQ;
+    }
+    if (nQ < 0) fail(FWN_INVALID_ARGUMENT, "negative point count");
+    if (nQ == 0) return;
+    if (nQ > (int64_t)UINT32_MAX / 2) fail(FWN_LIMIT_EXCEEDED, "more than 2^31 points in one call");
+    if (1 + (kWidth - 1) * (int64_t)e.max_depth > kStack)
+        fail(FWN_LIMIT_EXCEEDED, "tree deeper than the traversal stack supports");
+
+    std::vector<int64_t> boff(M + 1, 0);
+    for (int m = 0; m < M; ++m) boff[m + 1] = boff[m] + (qoff[m + 1] - qoff[m] + 63) / 64;
+    const int64_t nb = boff[M];
+
+    const float* dQ = Q;
+    float* dOm = omega;
+    uint8_t* dIn = inside;
+    int32_t* dSt = stats;
+    if (!dev) {
+        e.q_in.reserve(sizeof(float) * 3 * nQ);
+        e.q_om.reserve(sizeof(float) * nQ);
+        FWN_HIP(hipMemcpyAsync(e.q_in.p, Q, sizeof(float) * 3 * nQ, hipMemcpyHostToDevice, s));
+        dQ = e.q_in.as<float>();
+        dOm = e.q_om.as<float>();
+        if (inside) {
+            e.q_in8.reserve(nQ);
+            dIn = e.q_in8.as<uint8_t>();
+        }
+        if (stats) {
+            e.q_stats.reserve(sizeof(int32_t) * 4 * nQ);
+            dSt = e.q_stats.as<int32_t>();
+        }
+    }
+    e.q_off.reserve(sizeof(int64_t) * (M + 1));
+    e.q_boff.reserve(sizeof(int64_t) * (M + 1));
+    e.q_flag.reserve(sizeof(int32_t));
+    FWN_HIP(hipMemcpyAsync(e.q_off.p, qoff.data(), sizeof(int64_t) * (M + 1), hipMemcpyHostToDevice, s));
+    FWN_HIP(hipMemcpyAsync(e.q_boff.p, boff.data(), sizeof(int64_t) * (M + 1), hipMemcpyHostToDevice, s));
+
+    for (int k = 0; k < 2; ++k) {
+        e.q_keys[k].reserve(sizeof(uint64_t) * nQ);
+        e.q_vals[k].reserve(sizeof(uint32_t) * nQ);
+    }
+    if (!e.ev[0])
+        for (int k = 0; k < 4; ++k) FWN_HIP(hipEventCreate(&e.ev[k]));
+
+    FWN_HIP(hipEventRecord(e.ev[0], s));
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, s, e.q_flag.as<int32_t>());
+    uint64_t* kp = e.q_keys[0].as<uint64_t>();
+    uint32_t* vp = e.q_vals[0].as<uint32_t>();
+    uint64_t* kp2 = e.q_keys[1].as<uint64_t>();
+    uint32_t* vp2 = e.q_vals[1].as<uint32_t>();
+    hipLaunchKernelGGL(k_query_keys, dim3(grid_blocks(nQ, QT)), dim3(QT), 0, s, dQ, nQ, e.q_off.as<int64_t>(), M,
+                       e.mesh_info.as<float>(), kp, vp);
+    FWN_LAUNCH_CHECK("k_query_keys");
+    int mesh_bits = 0;
+    while ((1 << mesh_bits) < M) ++mesh_bits;
+    radix_sort_pairs(kp, vp, kp2, vp2, nQ, (M > 1) ? 32 + mesh_bits : 30, e.q_hist, e.q_rowsum, s);
+
+    constexpr int WAVES = 4;
+    const int64_t nblk = (nb + WAVES - 1) / WAVES;
+    if (nblk > INT32_MAX) fail(FWN_LIMIT_EXCEEDED, "too many point batches");
+    const float beta2 = e.opt.accuracy_scale * e.opt.accuracy_scale;
+    FWN_HIP(hipEventRecord(e.ev[1], s));
+    if (stats)
+        hipLaunchKernelGGL((k_traverse<true, WAVES>), dim3((unsigned)nblk), dim3(WAVES * 64), 0, s,
+                           e.slots.as<uint32_t>(), e.tris.as<uint32_t>(), e.roots.as<int32_t>(),
+                           e.mesh_info.as<float>(), dQ, vp, e.q_off.as<int64_t>(), e.q_boff.as<int64_t>(), M, nb,
+                           beta2, dOm, dIn, dSt, e.q_flag.as<int32_t>());
+    else
+        hipLaunchKernelGGL((k_traverse<false, WAVES>), dim3((unsigned)nblk), dim3(WAVES * 64), 0, s,
+                           e.slots.as<uint32_t>(), e.tris.as<uint32_t>(), e.roots.as<int32_t>(),
+                           e.mesh_info.as<float>(), dQ, vp, e.q_off.as<int64_t>(), e.q_boff.as<int64_t>(), M, nb,
+                           beta2, dOm, dIn, dSt, e.q_flag.as<int32_t>());
+    FWN_LAUNCH_CHECK("k_traverse");
+    FWN_HIP(hipEventRecord(e.ev[2], s));
+    int32_t h_flag = 0;
+    FWN_HIP(hipMemcpyAsync(&h_flag, e.q_flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    if (!dev) {
+        FWN_HIP(hipMemcpyAsync(omega, dOm, sizeof(float) * nQ, hipMemcpyDeviceToHost, s));
+        if (inside) FWN_HIP(hipMemcpyAsync(inside, dIn, nQ, hipMemcpyDeviceToHost, s));
+        if (stats) FWN_HIP(hipMemcpyAsync(stats, dSt, sizeof(int32_t) * 4 * nQ, hipMemcpyDeviceToHost, s));
+    }
+    FWN_HIP(hipStreamSynchronize(s));
+    float t_tr = 0.0f, t_all = 0.0f;
+    FWN_HIP(hipEventElapsedTime(&t_tr, e.ev[1], e.ev[2]));
+    FWN_HIP(hipEventElapsedTime(&t_all, e.ev[0], e.ev[2]));
+    e.last_traverse_ms = t_tr;
+    e.last_total_ms = t_all;
+    if (h_flag) fail(FWN_LIMIT_EXCEEDED, "traversal stack overflow");
+}
+
+void run_bruteforce(Engine& e, const float* Q, int64_t nQ, double* out, bool dev)
+{
+    hipStream_t s = e.stream;
+    if (nQ <= 0) return;
+    const float* dQ = Q;
+    double* dOut = out;
+    if (!dev) {
+        e.q_in.reserve(sizeof(float) * 3 * nQ);
+        e.q_bf.reserve(sizeof(double) * nQ);
+        FWN_HIP(hipMemcpyAsync(e.q_in.p, Q, sizeof(float) * 3 * nQ, hipMemcpyHostToDevice, s));
+        dQ = e.q_in.as<float>();
+        dOut = e.q_bf.as<double>();
+    }
+    // mesh 0's triangles are the first f_off[1] entries of the sorted triangle array
+    const int64_t cnt = e.h_f_off.size() > 1 ? e.h_f_off[1] : 0;
+    hipLaunchKernelGGL(k_bruteforce, dim3(grid_blocks(nQ, 256)), dim3(256), 0, s, e.tris.as<uint32_t>(), (int64_t)0,
+                       cnt, dQ, nQ, dOut);
+    FWN_LAUNCH_CHECK("k_bruteforce");
+    if (!dev) FWN_HIP(hipMemcpyAsync(out, dOut, sizeof(double) * nQ, hipMemcpyDeviceToHost, s));
+    FWN_HIP(hipStreamSynchronize(s));
+}
+
+} // namespace fwn

@@ -1,0 +1,233 @@
+// Stable LSD radix sort of (uint64 key, uint32 value) pairs and an int32 exclusive scan,
+// written for gfx950 wave64.  Used for the Morton order of triangles (tree build) and of
+// query points (wave batches of nearby points).
+//
+// One pass = 8 key bits:
+//   hist:    each 256-thread block counts the digits of its 4096-key tile in LDS
+//   rows:    one block per digit scans that digit's per-block counts (digit-major layout)
+//   digits:  one block scans the 256 digit totals
+//   scatter: each block re-reads its tile in 16 rounds of 256 keys; inside a wave the rank of a
+//            key among equal digits comes from 8 ballots, across the block's 4 waves from LDS
+//            counts, and the running per-digit offset keeps the order stable.
+#include "fwn_internal.h"
+
+namespace fwn {
+
+namespace {
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ROUNDS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;
+constexpr int RS_WAVES = RS_THREADS / kWave;
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* __restrict__ keys, int64_t n, int shift,
+                                                        int32_t* __restrict__ hist, int nblocks)
+{
+    __shared__ int32_t cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        int64_t i = base + r * RS_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(uint32_t)(keys[i] >> shift) & 255u], 1);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// exclusive scan of one digit's row of per-block counts; row total -> rowsum[d]
+__global__ __launch_bounds__(256) void k_rs_rows(int32_t* __restrict__ hist, int nblocks,
+                                                 int32_t* __restrict__ rowsum)
+{
+    __shared__ int32_t part[256];
+    int32_t* row = hist + (int64_t)blockIdx.x * nblocks;
+    int32_t carry = 0;
+    for (int base = 0; base < nblocks; base += 256) {
+        int i = base + threadIdx.x;
+        int32_t v = (i < nblocks) ? row[i] : 0;
+        part[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            int32_t t = (threadIdx.x >= (unsigned)off) ? part[threadIdx.x - off] : 0;
+            __syncthreads();
+            part[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nblocks) row[i] = carry + part[threadIdx.x] - v;
+        carry += part[255];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) rowsum[blockIdx.x] = carry;
+}
+
+__global__ __launch_bounds__(256) void k_rs_digits(int32_t* __restrict__ rowsum)
+{
+    __shared__ int32_t part[256];
+    int32_t v = rowsum[threadIdx.x];
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        int32_t t = (threadIdx.x >= (unsigned)off) ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += t;
+        __syncthreads();
+    }
+    rowsum[threadIdx.x] = part[threadIdx.x] - v;
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __restrict__ kin,
+                                                           const uint32_t* __restrict__ vin,
+                                                           uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                           int64_t n, int shift, const int32_t* __restrict__ hist,
+                                                           const int32_t* __restrict__ digit_base, int nblocks)
+{
+    __shared__ int32_t running[256];
+    __shared__ int32_t wc[RS_WAVES][256];
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    running[tid] = hist[(int64_t)tid * nblocks + blockIdx.x] + digit_base[tid];
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        if (base + r * RS_THREADS >= n) break; // uniform across the block
+        for (int k = tid; k < RS_WAVES * 256; k += RS_THREADS) (&wc[0][0])[k] = 0;
+        int64_t i = base + r * RS_THREADS + tid;
+        bool valid = i < n;
+        uint64_t key = valid ? kin[i] : 0ull;
+        uint32_t val = valid ? vin[i] : 0u;
+        uint32_t d = (uint32_t)(key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < 8; ++b) {
+            bool bit = (d >> b) & 1u;
+            uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        int rank = __popcll(peers & lt);
+        __syncthreads();
+        if (valid && rank == 0) wc[wave][d] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            int32_t pos = running[d] + rank;
+            for (int w = 0; w < wave; ++w) pos += wc[w][d];
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+        int32_t add = 0;
+        for (int w = 0; w < RS_WAVES; ++w) add += wc[w][tid];
+        running[tid] += add;
+        __syncthreads();
+    }
+}
+
+constexpr int SC_THREADS = 256;
+constexpr int SC_ITEMS = 4;
+constexpr int SC_TILE = SC_THREADS * SC_ITEMS;
+
+__device__ int32_t block_excl_scan(int32_t v, int32_t* part, int32_t& total)
+{
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < SC_THREADS; off <<= 1) {
+        int32_t t = (threadIdx.x >= (unsigned)off) ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += t;
+        __syncthreads();
+    }
+    total = part[SC_THREADS - 1];
+    int32_t r = part[threadIdx.x] - v;
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(SC_THREADS) void k_scan_tiles(const int32_t* __restrict__ in, int32_t* __restrict__ out,
+                                                           int64_t n, int32_t* __restrict__ tile_sums)
+{
+    __shared__ int32_t part[SC_THREADS];
+    int64_t base = (int64_t)blockIdx.x * SC_TILE + (int64_t)threadIdx.x * SC_ITEMS;
+    int32_t v[SC_ITEMS];
+    int32_t s = 0;
+    for (int k = 0; k < SC_ITEMS; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : 0;
+        s += v[k];
+    }
+    int32_t total;
+    int32_t pre = block_excl_scan(s, part, total);
+    for (int k = 0; k < SC_ITEMS; ++k) {
+        if (base + k < n) out[base + k] = pre;
+        pre += v[k];
+    }
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(SC_THREADS) void k_scan_sums(int32_t* __restrict__ sums, int64_t n)
+{
+    __shared__ int32_t part[SC_THREADS];
+    int32_t carry = 0;
+    for (int64_t base = 0; base < n; base += SC_THREADS) {
+        int64_t i = base + threadIdx.x;
+        int32_t v = (i < n) ? sums[i] : 0;
+        int32_t total;
+        int32_t pre = block_excl_scan(v, part, total);
+        if (i < n) sums[i] = carry + pre;
+        carry += total;
+    }
+    if (threadIdx.x == 0) sums[n] = carry;
+}
+
+__global__ __launch_bounds__(SC_THREADS) void k_scan_add(int32_t* __restrict__ out, int64_t n,
+                                                         const int32_t* __restrict__ tile_sums)
+{
+    int64_t base = (int64_t)blockIdx.x * SC_TILE + (int64_t)threadIdx.x * SC_ITEMS;
+    int32_t add = tile_sums[blockIdx.x];
+    for (int k = 0; k < SC_ITEMS; ++k)
+        if (base + k < n) out[base + k] += add;
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = tile_sums[gridDim.x];
+}
+
+} // namespace
+
+void radix_sort_pairs(uint64_t*& keys, uint32_t*& vals, uint64_t*& keys_alt, uint32_t*& vals_alt, int64_t n,
+                      int nbits, DBuf& hist, DBuf& rowsum, hipStream_t s)
+{
+    if (n <= 1) return;
+    if (n > (int64_t)INT32_MAX) fail(FWN_LIMIT_EXCEEDED, "radix sort: more than 2^31-1 elements");
+    int nblocks = (int)((n + RS_TILE - 1) / RS_TILE);
+    hist.reserve(sizeof(int32_t) * 256 * (size_t)nblocks);
+    rowsum.reserve(sizeof(int32_t) * 256);
+    for (int shift = 0; shift < nbits; shift += 8) {
+        hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(RS_THREADS), 0, s, keys, n, shift, hist.as<int32_t>(),
+                           nblocks);
+        FWN_LAUNCH_CHECK("k_rs_hist");
+        hipLaunchKernelGGL(k_rs_rows, dim3(256), dim3(256), 0, s, hist.as<int32_t>(), nblocks,
+                           rowsum.as<int32_t>());
+        FWN_LAUNCH_CHECK("k_rs_rows");
+        hipLaunchKernelGGL(k_rs_digits, dim3(1), dim3(256), 0, s, rowsum.as<int32_t>());
+        FWN_LAUNCH_CHECK("k_rs_digits");
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(RS_THREADS), 0, s, keys, vals, keys_alt, vals_alt, n,
+                           shift, hist.as<int32_t>(), rowsum.as<int32_t>(), nblocks);
+        FWN_LAUNCH_CHECK("k_rs_scatter");
+        std::swap(keys, keys_alt);
+        std::swap(vals, vals_alt);
+    }
+}
+
+// out must hold n+1 entries; out[n] receives the total.
+void exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, DBuf& tmp, hipStream_t s)
+{
+    if (n <= 0) {
+        FWN_HIP(hipMemsetAsync(out, 0, sizeof(int32_t), s));
+        return;
+    }
+    int64_t ntiles = (n + SC_TILE - 1) / SC_TILE;
+    tmp.reserve(sizeof(int32_t) * (size_t)(ntiles + 1));
+    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(SC_THREADS), 0, s, in, out, n, tmp.as<int32_t>());
+    FWN_LAUNCH_CHECK("k_scan_tiles");
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SC_THREADS), 0, s, tmp.as<int32_t>(), ntiles);
+    FWN_LAUNCH_CHECK("k_scan_sums");
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)ntiles), dim3(SC_THREADS), 0, s, out, n, tmp.as<int32_t>());
+    FWN_LAUNCH_CHECK("k_scan_add");
+}
+
+} // namespace fwn

@@ -1,0 +1,231 @@
+"""Host-side mirror of ``lagrange::winding::FastWindingNumber`` over the GPU engine.
+
+Reference API (modules/winding/include/lagrange/winding/FastWindingNumber.h:24-108):
+  * constructed from a triangle mesh; a non-3D or non-triangle mesh raises
+    (FastWindingNumber.cpp:202-207, la_runtime_assert -> lagrange::Error);
+  * positions are converted to float and indices to int (FastWindingNumber.cpp:150-163);
+  * ``solid_angle(pos)`` returns the solid angle in steradians (FastWindingNumber.cpp:180-187);
+  * ``is_inside(pos)`` is ``solid_angle / (4 pi) > 0.5`` evaluated in double
+    (FastWindingNumber.cpp:177);
+  * the object is move-only (copy constructor and assignment deleted, :73,:82).
+Additive here: batched calls over arrays (host numpy or device torch tensors) and the
+multi-mesh engine ``MultiFastWindingNumber``.  Every call runs on the GPU; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+class LagrangeError(RuntimeError):
+    """Counterpart of lagrange::Error (modules/core/include/lagrange/utils/Error.h:26)."""
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+def _mesh_arrays(mesh):
+    if isinstance(mesh, tuple) and len(mesh) == 2:
+        V, F = mesh
+    else:
+        V, F = mesh.vertices, mesh.facets
+    V = np.asarray(V)
+    F = np.asarray(F)
+    if V.ndim != 2 or V.shape[1] != 3:
+        raise LagrangeError("Fast winding number engine only supports 3D meshes")
+    if F.ndim != 2 or F.shape[1] != 3:
+        raise LagrangeError("Fast winding number engine only supports triangle meshes")
+    # the reference's internal copy: float positions, int indices (FastWindingNumber.cpp:150-163)
+    return (np.ascontiguousarray(V, dtype=np.float32),
+            np.ascontiguousarray(F.astype(np.int64, copy=False).astype(np.int32), dtype=np.int32))
+
+
+def _ptr(a) -> int:
+    return a.ctypes.data if isinstance(a, np.ndarray) else int(a.data_ptr())
+
+
+class _EngineBase:
+    _handle = None
+
+    def __copy__(self):
+        raise TypeError("FastWindingNumber is move-only (copy is deleted in the reference API)")
+
+    __deepcopy__ = lambda self, memo: self.__copy__()  # noqa: E731
+
+    def close(self):
+        if self._handle:
+            L.load().fwn_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _h(self):
+        if not self._handle:
+            raise LagrangeError("engine is not initialised (default-constructed or closed)")
+        return self._handle
+
+    # ---- introspection ----------------------------------------------------------------
+    def tree_info(self) -> dict:
+        info = L.TreeInfo()
+        L.check(L.load().fwn_tree_info_get(self._h(), C.byref(info)))
+        return {k: getattr(info, k) for k, _ in L.TreeInfo._fields_}
+
+    def export_tree(self):
+        """(slots u32 [nodes*8*32], tris u32 [F*12], roots int32 [M,2], tau float32 [M])."""
+        info = self.tree_info()
+        slots = np.zeros(max(info["num_nodes"], 1) * L.WIDTH * L.SLOT_WORDS, np.uint32)
+        tris = np.zeros(max(info["num_triangles"], 1) * L.TRI_WORDS, np.uint32)
+        roots = np.zeros((info["num_meshes"], 2), np.int32)
+        tau = np.zeros(info["num_meshes"], np.float32)
+        L.check(L.load().fwn_export_tree(self._h(), slots.ctypes.data, tris.ctypes.data, roots.ctypes.data,
+                                         tau.ctypes.data))
+        return slots, tris, roots, tau
+
+    def last_timing(self):
+        tr, tot = C.c_double(), C.c_double()
+        L.check(L.load().fwn_last_timing(self._h(), C.byref(tr), C.byref(tot)))
+        return tr.value, tot.value
+
+
+class FastWindingNumber(_EngineBase):
+    """Fast winding number engine for one triangle mesh (reference-compatible API)."""
+
+    def __init__(self, mesh=None, *, options: L.Options | None = None, **opts):
+        self._handle = None
+        if mesh is None:  # default constructor: an empty engine
+            return
+        V, F = _mesh_arrays(mesh)
+        o = options if options is not None else L.default_options(**opts)
+        h = C.c_void_p()
+        L.check(L.load().fwn_create(C.byref(h), V.ctypes.data, V.shape[0], F.ctypes.data, F.shape[0], C.byref(o)))
+        self._handle = h
+        self.num_vertices = V.shape[0]
+        self.num_facets = F.shape[0]
+
+    # ---- per-point API (as the reference) ------------------------------------------------
+    def solid_angle(self, pos) -> float:
+        q = np.ascontiguousarray(np.asarray(pos, dtype=np.float32).reshape(3))
+        out = C.c_float()
+        L.check(L.load().fwn_solid_angle_point(self._h(), q.ctypes.data, C.byref(out)))
+        return float(out.value)
+
+    def is_inside(self, pos) -> bool:
+        q = np.ascontiguousarray(np.asarray(pos, dtype=np.float32).reshape(3))
+        out = C.c_uint8()
+        L.check(L.load().fwn_is_inside_point(self._h(), q.ctypes.data, C.byref(out)))
+        return bool(out.value)
+
+    # ---- batched API -----------------------------------------------------------------------
+    def solid_angle_batch(self, points, with_inside: bool = False):
+        """Solid angles of an (N, 3) array; numpy in -> numpy out, torch device tensor in ->
+        torch device tensors out (no host round trip)."""
+        if _is_torch(points) and points.is_cuda:
+            import torch
+            q = points.contiguous().to(torch.float32)
+            n = q.shape[0]
+            om = torch.empty(n, dtype=torch.float32, device=q.device)
+            ins = torch.empty(n, dtype=torch.uint8, device=q.device) if with_inside else None
+            torch.cuda.synchronize(q.device)
+            L.check(L.load().fwn_solid_angle(self._h(), _ptr(q), n, _ptr(om), _ptr(ins) if ins is not None else None,
+                                             L.FWN_DEVICE_POINTERS))
+            return (om, ins.bool()) if with_inside else om
+        q = np.ascontiguousarray(np.asarray(points, dtype=np.float32).reshape(-1, 3))
+        n = q.shape[0]
+        om = np.empty(n, np.float32)
+        ins = np.empty(n, np.uint8) if with_inside else None
+        L.check(L.load().fwn_solid_angle(self._h(), q.ctypes.data, n, om.ctypes.data,
+                                         ins.ctypes.data if ins is not None else None, 0))
+        return (om, ins.astype(bool)) if with_inside else om
+
+    def is_inside_batch(self, points):
+        return self.solid_angle_batch(points, with_inside=True)[1]
+
+    def query_stats(self, points):
+        """(omega, stats[N,4]) with per-point work counters: tests, far terms, FP32 and FP64 triangles."""
+        q = np.ascontiguousarray(np.asarray(points, dtype=np.float32).reshape(-1, 3))
+        om = np.empty(q.shape[0], np.float32)
+        st = np.empty((q.shape[0], 4), np.int32)
+        L.check(L.load().fwn_query_stats(self._h(), q.ctypes.data, None, q.shape[0], om.ctypes.data,
+                                         st.ctypes.data, 0))
+        return om, st
+
+    def bruteforce_f64(self, points):
+        """FP64 sum of exact triangle solid angles, computed on the GPU (verification hook)."""
+        if _is_torch(points) and points.is_cuda:
+            import torch
+            q = points.contiguous().to(torch.float32)
+            out = torch.empty(q.shape[0], dtype=torch.float64, device=q.device)
+            torch.cuda.synchronize(q.device)
+            L.check(L.load().fwn_bruteforce_f64(self._h(), _ptr(q), q.shape[0], _ptr(out), L.FWN_DEVICE_POINTERS))
+            return out
+        q = np.ascontiguousarray(np.asarray(points, dtype=np.float32).reshape(-1, 3))
+        out = np.empty(q.shape[0], np.float64)
+        L.check(L.load().fwn_bruteforce_f64(self._h(), q.ctypes.data, q.shape[0], out.ctypes.data, 0))
+        return out
+
+
+class MultiFastWindingNumber(_EngineBase):
+    """Many meshes in one engine: one build and one query launch for all of them."""
+
+    def __init__(self, meshes, *, options: L.Options | None = None, **opts):
+        self._handle = None
+        arrs = [_mesh_arrays(m) for m in meshes]
+        if not arrs:
+            raise LagrangeError("at least one mesh is required")
+        V = np.ascontiguousarray(np.concatenate([a[0] for a in arrs]), dtype=np.float32)
+        F = np.ascontiguousarray(np.concatenate([a[1] for a in arrs]), dtype=np.int32)
+        self.v_off = np.zeros(len(arrs) + 1, np.int64)
+        self.f_off = np.zeros(len(arrs) + 1, np.int64)
+        self.v_off[1:] = np.cumsum([a[0].shape[0] for a in arrs])
+        self.f_off[1:] = np.cumsum([a[1].shape[0] for a in arrs])
+        o = options if options is not None else L.default_options(**opts)
+        h = C.c_void_p()
+        L.check(L.load().fwn_create_multi(C.byref(h), len(arrs), V.ctypes.data, self.v_off.ctypes.data,
+                                          F.ctypes.data, self.f_off.ctypes.data, C.byref(o)))
+        self._handle = h
+        self.num_meshes = len(arrs)
+
+    def solid_angle_batch(self, points_per_mesh=None, *, points=None, q_off=None, with_inside=False):
+        """Either a list of (Ni, 3) arrays (one per mesh) or one concatenated array + offsets."""
+        if points is None:
+            q_off = np.zeros(self.num_meshes + 1, np.int64)
+            q_off[1:] = np.cumsum([np.asarray(p).reshape(-1, 3).shape[0] for p in points_per_mesh])
+            if _is_torch(points_per_mesh[0]):
+                import torch
+                points = torch.cat([p.reshape(-1, 3) for p in points_per_mesh])
+            else:
+                points = np.concatenate([np.asarray(p, np.float32).reshape(-1, 3) for p in points_per_mesh])
+        q_off = np.ascontiguousarray(q_off, dtype=np.int64)
+        if _is_torch(points) and points.is_cuda:
+            import torch
+            q = points.contiguous().to(torch.float32)
+            n = q.shape[0]
+            om = torch.empty(n, dtype=torch.float32, device=q.device)
+            ins = torch.empty(n, dtype=torch.uint8, device=q.device) if with_inside else None
+            torch.cuda.synchronize(q.device)
+            L.check(L.load().fwn_solid_angle_multi(self._h(), _ptr(q), q_off.ctypes.data, _ptr(om),
+                                                   _ptr(ins) if ins is not None else None, L.FWN_DEVICE_POINTERS))
+            return (om, ins.bool()) if with_inside else om
+        q = np.ascontiguousarray(np.asarray(points, dtype=np.float32).reshape(-1, 3))
+        om = np.empty(q.shape[0], np.float32)
+        ins = np.empty(q.shape[0], np.uint8) if with_inside else None
+        L.check(L.load().fwn_solid_angle_multi(self._h(), q.ctypes.data, q_off.ctypes.data, om.ctypes.data,
+                                               ins.ctypes.data if ins is not None else None, 0))
+        return (om, ins.astype(bool)) if with_inside else om
+
+    def query_stats(self, points, q_off):
+        q = np.ascontiguousarray(np.asarray(points, dtype=np.float32).reshape(-1, 3))
+        q_off = np.ascontiguousarray(q_off, dtype=np.int64)
+        om = np.empty(q.shape[0], np.float32)
+        st = np.empty((q.shape[0], 4), np.int32)
+        L.check(L.load().fwn_query_stats(self._h(), q.ctypes.data, q_off.ctypes.data, q.shape[0], om.ctypes.data,
+                                         st.ctypes.data, 0))
+        return om, st

@@ -1,0 +1,289 @@
+"""GPU parity tests of the exact engine, all through the C ABI (ctypes).
+
+Checkers: the FP64 brute force (oracle.bruteforce), the CPU restatement of the reference
+engine (oracle.Restated), the FP32 same-tree traversal (oracle.tree_eval) and direct FP64
+moments (oracle.moments).  Tolerances:
+  * same tree, CPU vs GPU: |d omega| <= 2e-4 sr and identical per-point work counters (the
+    far/near decisions use the same float operations; far terms and atan2 differ by ulps);
+  * vs the FP64 brute force at the reference's default accuracy (beta = 2, order 2):
+    max |dw| <= 4e-3, mean <= 1e-3.  This is the truncation error of the order-2 expansion
+    for nodes just past the beta = 2 boundary (up to ~2e-4 per node, same sign for a point
+    near a smooth surface); the CPU restatement of the reference shows the same bound;
+  * vs the CPU restatement (different tree): max |dw| <= 4e-3;
+  * is_inside agrees with the brute force except within the error band around w = 0.5;
+  * integer work (Morton order, tree structure) is bit-exact.
+"""
+import numpy as np
+import pytest
+
+from lagrange_amd import workloads as W
+from lagrange_amd import _lib as L
+from lagrange_amd.winding import FastWindingNumber, MultiFastWindingNumber, LagrangeError
+
+pytestmark = pytest.mark.gpu
+
+FOUR_PI = 4.0 * np.pi
+
+
+def _wf(x):
+    return np.asarray(x, np.float64) / FOUR_PI
+
+
+def slot_view(slots):
+    s = slots.reshape(-1, 32)
+    return s.view(np.float32), s.view(np.int32)
+
+
+def test_sorted_triangle_order_is_morton_order(oracle, ico5):
+    V, F = ico5
+    e = FastWindingNumber((V, F))
+    _, tris, _, _ = e.export_tree()
+    got = tris.reshape(-1, 12)[:, 9].astype(np.int64)
+    want, _ = oracle.lbvh_sorted_order([V], [F])
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("leaf_max", [1, 4])
+def test_tree_structure_and_moments(oracle, leaf_max):
+    m = W.displaced_sphere(12, 5, delete_fraction=0.05)
+    V, F = m.f32()
+    e = FastWindingNumber((V, F), leaf_max=leaf_max)
+    info = e.tree_info()
+    slots, tris, roots, tau = e.export_tree()
+    fl, it = slot_view(slots)
+    tri_ids = tris.reshape(-1, 12)[:, 9].astype(np.int64)
+    root, rns = roots[0]
+    assert root == 0 and 1 <= rns <= 8
+    # structure: slots of a node tile its range contiguously; leaves tile [0, F)
+    covered = np.zeros(F.shape[0], np.int64)
+    stack = [(root, rns, 0, F.shape[0])]
+    depth_seen = 0
+    while stack:
+        node, ns, lo, cnt = stack.pop()
+        pos = lo
+        for s in range(ns):
+            k = node * 8 + s
+            first, count, child, cns = it[k, 28], it[k, 29], it[k, 27], it[k, 30]
+            assert first == pos and count >= 1
+            pos += count
+            if child < 0:
+                assert count <= leaf_max
+                covered[first:first + count] += 1
+            else:
+                assert 1 <= cns <= 8
+                stack.append((child, cns, first, count))
+        for s in range(ns, 8):
+            assert it[node * 8 + s, 29] == 0
+        assert pos == lo + cnt
+    assert np.all(covered == 1)
+    assert info["max_depth"] >= 1
+    # every occupied slot's data equals the FP64 moments of its triangles, computed directly
+    occupied = np.where(it[:, 29] > 0)[0]
+    for k in occupied:
+        first, count = it[k, 28], it[k, 29]
+        rec = oracle.moments(V, F, tri_ids[first:first + count])
+        want = _slot_words(rec)
+        got = fl[k, :27].astype(np.float64)
+        scale = np.abs(want).max()
+        err = np.abs(got - want)
+        err[3] = 0.0
+        assert err.max() <= 2e-5 * scale, (k, count, err.max() / scale)
+        # the radius bounds the distance from the stored centroid to every corner of the
+        # slot's exact box (so to every point of its triangles)
+        ids = tri_ids[first:first + count]
+        pts = V[F[ids].reshape(-1)].astype(np.float64)
+        blo, bhi = pts.min(0), pts.max(0)
+        p = got[0:3]
+        r2_exact = np.sum(np.maximum(np.abs(p - blo), np.abs(bhi - p)) ** 2)
+        assert got[3] >= r2_exact * (1 - 1e-7), (k, got[3], r2_exact)
+
+
+def _slot_words(rec):
+    M = rec[7:16]
+    S = rec[16:26]
+    f = np.zeros(27)
+    f[0:3] = rec[1:4]
+    f[3] = rec[26]
+    f[4:7] = rec[4:7]
+    f[7] = M[0] + M[4] + M[8]
+    f[8:14] = -3 * np.array([M[0], M[4], M[8], M[1] + M[3], M[5] + M[7], M[6] + M[2]])
+    f[14:24] = np.array([7.5] * 3 + [22.5] * 6 + [45.0]) * S
+    f[24] = -4.5 * (S[0] + S[5] + S[7])
+    f[25] = -4.5 * (S[3] + S[1] + S[8])
+    f[26] = -4.5 * (S[4] + S[6] + S[2])
+    return f
+
+
+def test_same_tree_parity_and_counters(oracle, open_mesh):
+    V, F = open_mesh
+    lo, hi = V.min(0), V.max(0)
+    q = np.concatenate([W.uniform_queries(lo, hi, 6000, 11, 1.3),
+                        W.near_surface_queries(W.Mesh(V.astype(np.float64), F.astype(np.int64)), 2000, 12)])
+    e = FastWindingNumber((V, F))
+    om, st = e.query_stats(q)
+    slots, tris, roots, tau = e.export_tree()
+    ref, rst = oracle.tree_eval(slots, 8, int(roots[0, 0]), int(roots[0, 1]), tris, q, 2.0, float(tau[0]), stats=True)
+    np.testing.assert_array_equal(st, rst)
+    assert np.abs(om.astype(np.float64) - ref).max() <= 2e-4
+    assert st[:, 3].sum() > 0  # the near-surface points exercised the FP64 path
+
+
+def test_vs_bruteforce_and_restated(oracle, ico5, open_mesh):
+    for V, F in (ico5, open_mesh):
+        lo, hi = V.min(0), V.max(0)
+        q = W.uniform_queries(lo, hi, 8000, 21, 1.3)
+        bf = oracle.bruteforce(V, F, q)
+        e = FastWindingNumber((V, F))
+        om, ins = e.solid_angle_batch(q, with_inside=True)
+        err = np.abs(_wf(om) - _wf(bf))
+        assert err.max() <= 4e-3 and err.mean() <= 1e-3, (err.max(), err.mean())
+        rs = oracle.Restated(V, F).solid_angle(q)
+        assert np.abs(_wf(om) - _wf(rs)).max() <= 4e-3
+        w = _wf(bf)
+        band = np.abs(w - 0.5) <= 1e-3 + err.max()
+        assert np.array_equal(ins[~band], (w > 0.5)[~band])
+        np.testing.assert_array_equal(ins, oracle.inside_from_omega(om))
+
+
+def test_gpu_bruteforce_f64(oracle, open_mesh):
+    V, F = open_mesh
+    q = W.uniform_queries(V.min(0), V.max(0), 512, 31, 1.2)
+    e = FastWindingNumber((V, F))
+    np.testing.assert_allclose(e.bruteforce_f64(q), oracle.bruteforce(V, F, q), rtol=0, atol=1e-9)
+
+
+def test_closed_mesh_inside_outside(ico5):
+    V, F = ico5
+    q = W.uniform_queries(np.full(3, -1.5), np.full(3, 1.5), 20000, 41)
+    r = np.linalg.norm(q.astype(np.float64), axis=1)
+    om = FastWindingNumber((V, F)).solid_angle_batch(q)
+    w = _wf(om)
+    assert np.abs(w[r < 0.9] - 1.0).max() <= 2e-3
+    assert np.abs(w[r > 1.1]).max() <= 2e-3
+
+
+def test_per_point_api_matches_batch(ico5):
+    V, F = ico5
+    e = FastWindingNumber((V, F))
+    pts = np.array([[0, 0, 0], [0.2, -0.3, 0.1], [2, 0, 0], [0.0, 0.0, 0.999]], np.float32)
+    om, ins = e.solid_angle_batch(pts, with_inside=True)
+    for p, o, i in zip(pts, om, ins):
+        assert e.solid_angle(p) == o
+        assert e.is_inside(p) == i
+    assert e.is_inside([0, 0, 0]) and not e.is_inside([2, 0, 0])
+
+
+def test_single_triangle_known_answers():
+    # octant triangle seen from the origin: +pi/2 (modules/conftest.py single_triangle geometry)
+    V = np.eye(3, dtype=np.float32)
+    e = FastWindingNumber((V, np.array([[0, 1, 2]], np.int32)))
+    assert abs(e.solid_angle([0, 0, 0]) - np.pi / 2) <= 1e-6
+    er = FastWindingNumber((V, np.array([[0, 2, 1]], np.int32)))
+    assert abs(er.solid_angle([0, 0, 0]) + np.pi / 2) <= 1e-6
+    # a query on a vertex contributes nothing
+    assert e.solid_angle([1, 0, 0]) == 0.0
+
+
+def test_cube_known_answers():
+    # unit cube of modules/conftest.py (cube_triangular): each face subtends 2pi/3 from the centre
+    V = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], np.float32)
+    F = np.array([[0, 3, 2], [2, 1, 0], [4, 5, 6], [6, 7, 4], [1, 2, 6], [6, 5, 1], [4, 7, 3], [3, 0, 4],
+                  [2, 3, 7], [7, 6, 2], [0, 1, 5], [5, 4, 0]], np.int32)
+    e = FastWindingNumber((V, F))
+    assert abs(e.solid_angle([0.5, 0.5, 0.5]) - 4 * np.pi) <= 1e-5
+    assert abs(e.solid_angle([1.5, 0.5, 0.5])) <= 1e-5
+    top = FastWindingNumber((V, F[2:4]))  # z = 1 face, outward +z
+    assert abs(top.solid_angle([0.5, 0.5, 0.5]) - 2 * np.pi / 3) <= 1e-5
+
+
+def test_edge_cases(ico5):
+    V, F = ico5
+    e = FastWindingNumber((V, F))
+    assert e.solid_angle_batch(np.zeros((0, 3), np.float32)).shape == (0,)
+    # duplicated points give identical results
+    q = np.repeat(np.array([[0.3, 0.1, -0.2]], np.float32), 130, axis=0)
+    om = e.solid_angle_batch(q)
+    assert np.all(om == om[0])
+    # empty mesh: every point has zero solid angle
+    empty = FastWindingNumber((np.zeros((0, 3), np.float32), np.zeros((0, 3), np.int32)))
+    assert np.all(empty.solid_angle_batch(q) == 0.0)
+    # degenerate (zero-area) triangles contribute nothing
+    Vd = np.array([[0, 0, 0], [1, 0, 0], [2, 0, 0]], np.float32)
+    ed = FastWindingNumber((Vd, np.array([[0, 1, 2], [0, 0, 1]], np.int32)))
+    assert np.all(ed.solid_angle_batch(np.array([[0.5, 1, 0], [5, 5, 5]], np.float32)) == 0.0)
+
+
+def test_invalid_inputs_raise():
+    V = np.eye(3, dtype=np.float32)
+    with pytest.raises(L.FWNError) as ex:
+        FastWindingNumber((V, np.array([[0, 1, 3]], np.int32)))
+    assert ex.value.status == L.FWN_INVALID_ARGUMENT
+    Vn = V.copy()
+    Vn[0, 0] = np.nan
+    with pytest.raises(L.FWNError):
+        FastWindingNumber((Vn, np.array([[0, 1, 2]], np.int32)))
+    with pytest.raises(LagrangeError):
+        FastWindingNumber((np.zeros((3, 2), np.float32), np.array([[0, 1, 2]], np.int32)))
+    with pytest.raises(LagrangeError):
+        FastWindingNumber((V, np.array([[0, 1, 2, 0]], np.int32)))
+    with pytest.raises(L.FWNError):
+        FastWindingNumber((V, np.array([[0, 1, 2]], np.int32)), leaf_max=9)
+    with pytest.raises(LagrangeError):
+        FastWindingNumber().solid_angle([0, 0, 0])
+
+
+def test_multi_mesh_equals_single_engines():
+    meshes = [W.icosphere(6), W.torus(40, 12, seed=3, amplitude=0.2),
+              W.Mesh(np.zeros((0, 3)), np.zeros((0, 3), np.int64)),
+              W.Mesh(np.eye(3), np.array([[0, 1, 2]])),
+              W.displaced_sphere(10, 9, delete_fraction=0.05)]
+    pts = []
+    for i, m in enumerate(meshes):
+        lo, hi = (m.bbox() if m.num_facets else (np.full(3, -1.0), np.full(3, 1.0)))
+        pts.append(W.uniform_queries(lo, hi, [1000, 77, 64, 5, 2500][i], 50 + i, 1.3))
+    mm = MultiFastWindingNumber([(m.vertices, m.facets) for m in meshes])
+    om = mm.solid_angle_batch(pts)
+    off = np.cumsum([0] + [p.shape[0] for p in pts])
+    for i, m in enumerate(meshes):
+        single = FastWindingNumber((m.vertices, m.facets)).solid_angle_batch(pts[i])
+        np.testing.assert_array_equal(om[off[i]:off[i + 1]], single)
+
+
+def test_torch_device_pointers(ico5):
+    torch = pytest.importorskip("torch")
+    V, F = ico5
+    e = FastWindingNumber((V, F))
+    q = W.uniform_queries(np.full(3, -1.5), np.full(3, 1.5), 5000, 61)
+    om_h = e.solid_angle_batch(q)
+    om_d, ins_d = e.solid_angle_batch(torch.from_numpy(q).cuda(), with_inside=True)
+    np.testing.assert_array_equal(om_d.cpu().numpy(), om_h)
+    np.testing.assert_array_equal(ins_d.cpu().numpy(), om_h.astype(np.float64) / (4 * np.pi) > 0.5)
+
+
+def test_large_mesh_invariants():
+    """~1M triangles, closed: size-independent inside/outside invariants on a 64^3 grid."""
+    m = W.icosphere(224)
+    V, F = m.f32()
+    e = FastWindingNumber((V, F))
+    q = W.grid_queries(V.min(0), V.max(0), 64, 1.2)
+    w = _wf(e.solid_angle_batch(q))
+    r = np.linalg.norm(q.astype(np.float64), axis=1)
+    assert np.abs(w[r < 0.95] - 1).max() <= 4e-3
+    assert np.abs(w[r > 1.05]).max() <= 4e-3
+    assert np.abs(w[r < 0.5] - 1).max() <= 1e-3
+    assert np.abs(w[r > 1.5]).max() <= 1e-3
+    info = e.tree_info()
+    assert info["num_triangles"] == 1_003_520
+
+
+def test_large_open_mesh_vs_gpu_bruteforce():
+    """The ~1M-triangle open displaced sphere vs the FP64 brute force on a point subsample."""
+    m = W.large_displaced_mesh()
+    V, F = m.f32()
+    e = FastWindingNumber((V, F))
+    q = W.grid_queries(V.min(0), V.max(0), 256, 1.2)
+    sub = q[np.arange(0, q.shape[0], 4099)]
+    w = _wf(e.solid_angle_batch(sub))
+    wb = _wf(e.bruteforce_f64(sub))
+    err = np.abs(w - wb)
+    assert err.max() <= 4e-3 and err.mean() <= 1e-3, (err.max(), err.mean())
