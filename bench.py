@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Benchmark of the exact winding-number engine on one MI355X.
+
+Workload: the ~1M-triangle open displaced sphere (nu = 224 geodesic sphere, 32 seeded spherical
+harmonics up to degree 16, amplitude 0.15, 5% of facets deleted -> 953,344 triangles) and a
+256^3 grid of points over its bounding box x1.2 (16,777,216 points).  One step = one batched
+solid_angle + is_inside call over all points (Morton sort of the points, traversal, outputs),
+with the points already resident in HBM.  The tree is built once before the timed region, as
+the reference benchmark constructs its engine before timing queries
+(modules/winding/tests/test_fast_winding_number.cpp:99-110).
+
+Also reported (outside the timed region):
+  * roofline of the traversal kernel: algorithmic FP32 FLOPs (per-point work counters of the
+    same tree and criterion x FLOPs per operation) / its average duration from HIP events on
+    the engine's stream;
+  * accuracy of the timed results against the engine's FP64 brute force on a strided sample;
+  * cpu_baseline: the CPU restatement of the reference engine (oracle/, OpenMP over points)
+    on the host cores, over a strided sample of the same points.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# FP32 FLOPs per operation of the traversal kernel, counted from fwn_query.hip
+# (FMA = 2, each rsqrt / atan2 counted as 1):
+#   slot test   : d = p - q (3), |d|^2 (5), beta^2 r2 (1), compare (1)                      = 10
+#   far term    : rsq + 4 powers (5), d.N (5), 6 products (6), order-1 (11 + 6),
+#                 10 monomials (10), order-2 (19), combine + accumulate (7)                  = 69
+#   FP32 tri    : offsets (9), lengths (15), 3 rsq (3), normalise (9), edges (6), cross (9),
+#                 triple product (5), dots (15), denominator (3), atan2 (1), x2 + acc (2)    = 77
+#   FP64 tri    : the same work in FP64 (reported separately, not in the FP32 figure)       = 77
+FLOP_TEST, FLOP_FAR, FLOP_TRI32, FLOP_TRI64 = 10, 69, 77, 77
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+METRIC = "winding queries/s @1M tris: exact GPU vs surrogate at ≤1e-4 inside-error"
+
+
+def _read_traffic():
+    """HBM bytes per traversal launch from the committed rocprofv3 --pmc summary, if any."""
+    p = os.path.join(ROOT, "profiles", "k_traverse_large_exact_pmc.txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(V, F, q, sample_stride: int, threads: int):
+    from oracle import oracle as O
+    sub = q[::sample_stride].copy()
+    t0 = time.perf_counter()
+    R = O.Restated(V, F)
+    build_s = time.perf_counter() - t0
+    R.solid_angle(sub[:4096], nthreads=threads)  # warm-up
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        R.solid_angle(sub, nthreads=threads)
+        times.append(time.perf_counter() - t0)
+    dt = sorted(times)[1]
+    return {
+        "value": sub.shape[0] / dt,
+        "unit": "queries/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"every {sample_stride}th grid point ({sub.shape[0]} points), median of 3 runs; "
+                  f"CPU restatement of the reference engine (4-ary box-area BVH, order 2, beta 2, FP32), "
+                  f"OpenMP over points; build {build_s:.2f} s single-threaded, not included",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--leaf-max", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-stride", type=int, default=4)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (scale for rocprofv3 FETCH/WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": a.numel() * 4, "launches": 3}))
+        return
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this engine's workloads run on a single MI355X "
+              "(query sharding across GPUs is a design note, not built); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from lagrange_amd import workloads as W
+    from lagrange_amd.winding import FastWindingNumber
+
+    mesh, q = W.large_exact_workload(grid_n=args.grid)
+    V, F = mesh.f32()
+    n = q.shape[0]
+    dev = torch.device("cuda:0")
+    e = FastWindingNumber((V, F), leaf_max=args.leaf_max)
+    info = e.tree_info()
+    qd = torch.from_numpy(q).to(dev)
+    torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        om, ins = e.solid_angle_batch(qd, with_inside=True)
+    torch.cuda.synchronize()
+    trav = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        om, ins = e.solid_angle_batch(qd, with_inside=True)
+        trav.append(e.last_timing()[0])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per_step = dt * 1e3 / args.steps
+    value = n * args.steps / dt
+
+    # algorithmic work of the traversal on this tree and criterion (all points, one extra launch)
+    _, st = e.query_stats(q)
+    tot = st.astype(np.int64).sum(axis=0)
+    flops32 = tot[0] * FLOP_TEST + tot[1] * FLOP_FAR + tot[2] * FLOP_TRI32
+    flops64 = tot[3] * FLOP_TRI64
+    trav_ms = float(np.mean(trav))
+    achieved = flops32 / (trav_ms * 1e-3) / 1e12
+    traffic = _read_traffic()
+
+    # accuracy of the timed results vs the engine's FP64 brute force on a strided sample
+    idx = np.arange(0, n, 4099)
+    w = om.cpu().numpy()[idx].astype(np.float64) / (4 * np.pi)
+    wb = e.bruteforce_f64(q[idx]) / (4 * np.pi)
+    err = np.abs(w - wb)
+    ins_ok = ins.cpu().numpy()[idx] == (wb > 0.5)
+    amb = np.abs(wb - 0.5) <= 1e-3
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "exact engine, displaced sphere nu=224 with 5% facets deleted "
+                        f"({F.shape[0]} triangles), {args.grid}^3 grid over bbox x1.2 ({n} points)",
+            "engine": "exact",
+            "accuracy_scale": 2.0,
+            "order": 2,
+            "leaf_max": args.leaf_max,
+            "tree_width": info["width"],
+            "tree_nodes": info["num_nodes"],
+            "tree_depth": info["max_depth"],
+            "build_ms": round(info["build_ms"], 3),
+            "fp64_near_field_rel": 1e-4,
+        },
+        "roofline": {
+            "bound": "valu",
+            "kernel": "k_traverse",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": traffic,
+            "kernel_ms": trav_ms,
+            "flop_per_point": flops32 / n,
+            "fp64_flop_per_point": flops64 / n,
+            "ops_per_point": {"tests": tot[0] / n, "far": tot[1] / n, "tri32": tot[2] / n, "tri64": tot[3] / n},
+            "hbm_algorithmic_gbs": n * 17 / (trav_ms * 1e-3) / 1e9,
+            "hbm_peak_gbs": PEAK_HBM_GBS,
+        },
+        "accuracy": {
+            "sample": int(idx.size),
+            "vs": "FP64 brute force (GPU kernel)",
+            "max_abs_dw": float(err.max()),
+            "p99_abs_dw": float(np.quantile(err, 0.99)),
+            "mean_abs_dw": float(err.mean()),
+            "inside_mismatch_excl_band": int((~ins_ok & ~amb).sum()),
+        },
+    }
+    if not args.no_cpu:
+        threads = len(os.sched_getaffinity(0))
+        threads = min(threads, 16)
+        line["cpu_baseline"] = cpu_baseline(V, F, q, args.cpu_stride, threads)
+        line["speedup_vs_cpu"] = value / line["cpu_baseline"]["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
