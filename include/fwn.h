@@ -54,19 +54,23 @@ typedef struct fwn_engine fwn_engine;
 
 typedef struct fwn_options {
     float accuracy_scale;     /* beta: far field when |q-p|^2 > beta^2 r^2. Reference default 2 */
-    float near_fp64_rel;      /* FP64 triangle solid angle when q is within this x bbox-diagonal of
-                                 the leaf's bounding sphere (default 1e-4); 0 disables FP64 */
+    float near_fp64_rel;      /* tau = this x bbox diagonal (default 1e-4; 0 disables FP64): a leaf's
+                                 triangles use the FP64 solid angle when q is within tau of the
+                                 leaf's bounding sphere and, for a single triangle, within tau of
+                                 its plane */
     int32_t leaf_max;         /* max triangles per leaf bucket (1..8, default 1: every triangle
                                  carries its own far-field expansion, as the reference's items do) */
     int32_t fp64_moment_min;  /* tree nodes with at least this many triangles merge moments in FP64 */
     int32_t device;           /* HIP device ordinal */
-    int32_t waves_per_block;  /* query launch shape (0 = autotuned/default) */
-    int32_t reserved[6];
+    int32_t waves_per_block;  /* query launch shape (0 = default) */
+    int32_t tree_width;       /* slots per tree node: 2, 4 or 8 (default 4) */
+    int32_t reserved[5];
 } fwn_options;
 
 /* ---- exported tree format (fwn_export_tree) ------------------------------------------------
- * The engine stores, per mesh, a wide tree of FWN_WIDTH-slot nodes.  Each slot is 32 32-bit
- * words.  Words 0..26 are float, 27..31 int32:
+ * The engine stores, per mesh, a tree of nodes with tree_width (<= FWN_WIDTH) slots each; node
+ * n's slot s is at word (n * tree_width + s) * FWN_SLOT_WORDS.  Each slot is 32 32-bit words.
+ * Words 0..26 are float, 27..31 int32:
  *   0-2  p     area-weighted centroid of the slot's triangles
  *   3    r2    squared radius bound: every point of the slot's triangles is within sqrt(r2) of p
  *   4-6  N     sum of area-weighted normals
@@ -83,7 +87,7 @@ typedef struct fwn_options {
  * Triangles are stored in tree order, FWN_TRI_WORDS words each: v0, v1, v2 (9 floats),
  * original facet index (int32), 2 reserved words.
  */
-#define FWN_WIDTH 8
+#define FWN_WIDTH 8 /* maximum tree_width */
 #define FWN_SLOT_WORDS 32
 #define FWN_TRI_WORDS 12
 
@@ -100,7 +104,7 @@ typedef struct fwn_tree_info {
     int64_t num_meshes;
     int64_t num_nodes;       /* wide nodes over all meshes */
     int64_t num_triangles;   /* over all meshes */
-    int32_t width;           /* FWN_WIDTH */
+    int32_t width;           /* slots per node (tree_width) */
     int32_t max_depth;       /* deepest wide-node level (root = 1) */
     double build_ms;         /* device time of the last build */
 } fwn_tree_info;
@@ -142,13 +146,14 @@ int fwn_is_inside_point(fwn_engine* e, const float q[3], uint8_t* inside_out);
 /* ---- measurement / verification hooks ---------------------------------------------------- */
 int fwn_tree_info_get(const fwn_engine* e, fwn_tree_info* info);
 
-/* Copy the tree to host memory.  slots: num_nodes*FWN_WIDTH*FWN_SLOT_WORDS words; tris:
+/* Copy the tree to host memory.  slots: num_nodes*width*FWN_SLOT_WORDS words; tris:
  * num_triangles*FWN_TRI_WORDS words; roots: 2 int32 per mesh (root node, occupied slots);
  * tau: one float per mesh (absolute FP64 near-field distance).  Any pointer may be NULL. */
 int fwn_export_tree(const fwn_engine* e, uint32_t* slots, uint32_t* tris, int32_t* roots, float* tau);
 
 /* Same traversal as fwn_solid_angle (single mesh: q_off NULL) but also writes FWN_STAT_COUNT
- * int32 work counters per point (stats_out, host or device per flags). */
+ * int32 work counters per point (stats_out, host or device per flags): slots tested, far-field
+ * terms, FP32 triangles, FP64 triangles. */
 int fwn_query_stats(fwn_engine* e, const float* Q, const int64_t* q_off, int64_t nQ, float* omega_out,
                     int32_t* stats_out, int32_t flags);
 

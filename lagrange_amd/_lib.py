@@ -23,7 +23,7 @@ FWN_HIP_ERROR = 4
 FWN_LIMIT_EXCEEDED = 5
 FWN_DEVICE_POINTERS = 1
 
-WIDTH = 8
+MAX_WIDTH = 8
 SLOT_WORDS = 32
 TRI_WORDS = 12
 
@@ -39,7 +39,8 @@ class Options(C.Structure):
         ("fp64_moment_min", C.c_int32),
         ("device", C.c_int32),
         ("waves_per_block", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("tree_width", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
 

@@ -72,5 +72,37 @@ def _drain(procs):
         raise RuntimeError("hipcc failed:\n" + "\n".join(errs))
 
 
+CXX_DIR = os.path.join(HERE, "cpp")
+CXX_LIB = os.path.join(HERE, "liblagrange_winding.so")
+CXX_TEST = os.path.join(CXX_DIR, "build", "test_fast_winding_number")
+
+
+def build_cxx(verbose: bool = False) -> str:
+    """The lagrange::winding::FastWindingNumber drop-in (g++, links libfwn.so) and its test runner.
+
+    Uses the compat Lagrange headers under cpp/compat; a build against real Lagrange puts its
+    include directory first instead."""
+    inc = ["-I", os.path.join(CXX_DIR, "include"), "-I", os.path.join(CXX_DIR, "compat"),
+           "-I", os.path.join(ROOT, "include")]
+    link = ["-L", HERE, "-lfwn", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+    cmds = [
+        ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-o", CXX_LIB,
+         os.path.join(CXX_DIR, "src", "FastWindingNumber.cpp")] + inc + link,
+    ]
+    os.makedirs(os.path.dirname(CXX_TEST), exist_ok=True)
+    cmds.append(["g++", "-std=c++17", "-O2", "-Wall", "-pthread", "-o", CXX_TEST,
+                 os.path.join(CXX_DIR, "tests", "test_fast_winding_number.cpp")] + inc +
+                ["-L", HERE, "-llagrange_winding", "-lfwn", "-Wl,-rpath," + HERE,
+                 "-Wl,-rpath-link,/opt/rocm/lib"])
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("g++ failed:\n" + r.stdout)
+    return CXX_LIB
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_cxx(verbose="-v" in sys.argv))

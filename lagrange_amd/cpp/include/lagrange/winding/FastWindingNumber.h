@@ -1,0 +1,62 @@
+// lagrange::winding::FastWindingNumber backed by the MI355X engine (include/fwn.h).
+//
+// The class keeps the reference declaration (modules/winding/include/lagrange/winding/
+// FastWindingNumber.h:24-108): constructor from SurfaceMesh<Scalar, Index> (float/double x
+// uint32/uint64), default constructor, move-only, const is_inside/solid_angle per point, PIMPL
+// through value_ptr<Impl>.  The batched overloads below are additions; they take plain arrays of
+// points and run one GPU launch for all of them.
+#pragma once
+
+#include <lagrange/SurfaceMesh.h>
+
+#include <array>
+#include <cstddef>
+#include <cstdint>
+
+namespace lagrange {
+
+namespace winding {
+
+///
+/// Fast winding number computation for triangle soups.
+///
+class FastWindingNumber
+{
+public:
+    ///
+    /// Builds the acceleration structure of a triangle mesh (positions converted to float and
+    /// indices to int, as the reference engine does).  Throws lagrange::Error if the mesh is not
+    /// 3D, not a triangle mesh, or the GPU engine cannot be created.
+    ///
+    template <typename Scalar, typename Index>
+    FastWindingNumber(const SurfaceMesh<Scalar, Index>& mesh);
+
+    FastWindingNumber();
+    ~FastWindingNumber();
+    FastWindingNumber(FastWindingNumber&& other) noexcept;
+    FastWindingNumber& operator=(FastWindingNumber&& other) noexcept;
+    FastWindingNumber(const FastWindingNumber& other) = delete;
+    FastWindingNumber& operator=(const FastWindingNumber& other) = delete;
+
+    /// True if the winding number at pos exceeds 1/2 (solid angle / 4 pi > 0.5, in double).
+    bool is_inside(const std::array<float, 3>& pos) const;
+
+    /// Solid angle (steradians) at pos.
+    float solid_angle(const std::array<float, 3>& pos) const;
+
+    /// Batched solid angles of n points (row-major n x 3); out has n entries.
+    void solid_angle(const std::array<float, 3>* pos, size_t n, float* out) const;
+
+    /// Batched inside tests of n points; out has n entries.
+    void is_inside(const std::array<float, 3>* pos, size_t n, bool* out) const;
+
+protected:
+    /// Internal implementation.
+    struct Impl;
+
+    /// PIMPL to hide internal data structures.
+    value_ptr<Impl> m_impl;
+};
+
+} // namespace winding
+} // namespace lagrange

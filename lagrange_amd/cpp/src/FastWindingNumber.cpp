@@ -1,0 +1,112 @@
+// lagrange::winding::FastWindingNumber over the engine's C ABI.
+//
+// Mirrors modules/winding/src/FastWindingNumber.cpp: the constructor asserts a 3D triangle mesh
+// (:202-207), copies positions to float and indices to int (:150-163) and builds the engine
+// (:168); is_inside is solid_angle / (4 pi) > 0.5 in double (:177).  Here the build and every
+// query run on the GPU through fwn_create / fwn_solid_angle*.
+#include <lagrange/winding/FastWindingNumber.h>
+
+#include <fwn.h>
+
+#include <string>
+#include <vector>
+
+namespace lagrange {
+namespace winding {
+
+namespace {
+[[noreturn]] void raise(const char* what, int status)
+{
+    throw Error(std::string(what) + " failed (status " + std::to_string(status) + "): " + fwn_last_error());
+}
+} // namespace
+
+struct FastWindingNumber::Impl
+{
+    fwn_engine* engine = nullptr;
+
+    Impl() = default;
+    Impl(const Impl&) = delete;
+    Impl& operator=(const Impl&) = delete;
+    ~Impl()
+    {
+        if (engine) fwn_destroy(engine);
+    }
+
+    template <typename VView, typename FView>
+    void initialize(const VView& vertices, const FView& facets)
+    {
+        if (vertices.cols() != 3) throw Error("Fast winding number engine only supports 3D meshes");
+        if (facets.cols() != 3) throw Error("Fast winding number engine only supports triangle meshes");
+        std::vector<float> v(static_cast<size_t>(vertices.rows()) * 3);
+        for (std::ptrdiff_t i = 0; i < (std::ptrdiff_t)vertices.rows(); ++i)
+            for (int d = 0; d < 3; ++d) v[3 * i + d] = static_cast<float>(vertices(i, d));
+        std::vector<int32_t> f(static_cast<size_t>(facets.rows()) * 3);
+        for (std::ptrdiff_t i = 0; i < (std::ptrdiff_t)facets.rows(); ++i)
+            for (int d = 0; d < 3; ++d) f[3 * i + d] = static_cast<int32_t>(facets(i, d));
+        int st = fwn_create(&engine, v.data(), (int64_t)vertices.rows(), f.data(), (int64_t)facets.rows(), nullptr);
+        if (st != FWN_OK) raise("fwn_create", st);
+    }
+
+    void check() const
+    {
+        if (!engine) throw Error("FastWindingNumber is not initialised");
+    }
+};
+
+template <typename Scalar, typename Index>
+FastWindingNumber::FastWindingNumber(const SurfaceMesh<Scalar, Index>& mesh) : m_impl(make_value_ptr<Impl>())
+{
+    if (mesh.get_dimension() != 3) throw Error("Fast winding number engine only supports 3D meshes");
+    if (!mesh.is_triangle_mesh()) throw Error("Fast winding number engine only supports triangle meshes");
+    m_impl->initialize(vertex_view(mesh), facet_view(mesh));
+}
+
+FastWindingNumber::FastWindingNumber() = default;
+FastWindingNumber::~FastWindingNumber() = default;
+FastWindingNumber::FastWindingNumber(FastWindingNumber&& other) noexcept = default;
+FastWindingNumber& FastWindingNumber::operator=(FastWindingNumber&& other) noexcept = default;
+
+bool FastWindingNumber::is_inside(const std::array<float, 3>& pos) const
+{
+    m_impl->check();
+    uint8_t r = 0;
+    int st = fwn_is_inside_point(m_impl->engine, pos.data(), &r);
+    if (st != FWN_OK) raise("fwn_is_inside_point", st);
+    return r != 0;
+}
+
+float FastWindingNumber::solid_angle(const std::array<float, 3>& pos) const
+{
+    m_impl->check();
+    float r = 0.0f;
+    int st = fwn_solid_angle_point(m_impl->engine, pos.data(), &r);
+    if (st != FWN_OK) raise("fwn_solid_angle_point", st);
+    return r;
+}
+
+void FastWindingNumber::solid_angle(const std::array<float, 3>* pos, size_t n, float* out) const
+{
+    m_impl->check();
+    int st = fwn_solid_angle(m_impl->engine, pos ? pos->data() : nullptr, (int64_t)n, out, nullptr, 0);
+    if (st != FWN_OK) raise("fwn_solid_angle", st);
+}
+
+void FastWindingNumber::is_inside(const std::array<float, 3>* pos, size_t n, bool* out) const
+{
+    m_impl->check();
+    std::vector<float> om(n);
+    std::vector<uint8_t> in(n);
+    int st = fwn_solid_angle(m_impl->engine, pos ? pos->data() : nullptr, (int64_t)n, om.data(), in.data(), 0);
+    if (st != FWN_OK) raise("fwn_solid_angle", st);
+    for (size_t i = 0; i < n; ++i) out[i] = in[i] != 0;
+}
+
+// float/double x uint32/uint64, as LA_SURFACE_MESH_X (modules/core/include/lagrange/SurfaceMeshTypes.h:49-53)
+template FastWindingNumber::FastWindingNumber(const SurfaceMesh<float, uint32_t>&);
+template FastWindingNumber::FastWindingNumber(const SurfaceMesh<double, uint32_t>&);
+template FastWindingNumber::FastWindingNumber(const SurfaceMesh<float, uint64_t>&);
+template FastWindingNumber::FastWindingNumber(const SurfaceMesh<double, uint64_t>&);
+
+} // namespace winding
+} // namespace lagrange

@@ -1,0 +1,184 @@
+// Tests of lagrange::winding::FastWindingNumber (drop-in class over the GPU engine).
+// Self-contained runner: prints one line per case and returns non-zero on failure.
+// Geometry: the unit cube and octant triangle of the reference's pytest fixtures
+// (modules/conftest.py:29-37, 108-144) and a subdivided octahedron projected to the sphere.
+#include <lagrange/winding/FastWindingNumber.h>
+
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <functional>
+#include <map>
+#include <string>
+#include <thread>
+#include <utility>
+#include <vector>
+
+using lagrange::SurfaceMesh;
+using lagrange::winding::FastWindingNumber;
+
+static int g_fail = 0;
+#define CHECK(cond)                                                                   \
+    do {                                                                              \
+        if (!(cond)) {                                                                \
+            std::printf("  CHECK failed: %s (%s:%d)\n", #cond, __FILE__, __LINE__);  \
+            ++g_fail;                                                                 \
+        }                                                                             \
+    } while (0)
+
+static const double kPi = 3.14159265358979323846;
+
+template <typename S, typename I>
+SurfaceMesh<S, I> cube()
+{
+    SurfaceMesh<S, I> m;
+    const S v[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+    for (auto& p : v) m.add_vertex(p);
+    const I f[12][3] = {{0, 3, 2}, {2, 1, 0}, {4, 5, 6}, {6, 7, 4}, {1, 2, 6}, {6, 5, 1},
+                        {4, 7, 3}, {3, 0, 4}, {2, 3, 7}, {7, 6, 2}, {0, 1, 5}, {5, 4, 0}};
+    for (auto& t : f) m.add_triangle(t[0], t[1], t[2]);
+    return m;
+}
+
+// octahedron with each face split into n^2 triangles, projected to the unit sphere
+SurfaceMesh<double, uint32_t> sphere(int n)
+{
+    SurfaceMesh<double, uint32_t> m;
+    const double P[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    const int F[8][3] = {{0, 2, 4}, {2, 1, 4}, {1, 3, 4}, {3, 0, 4}, {2, 0, 5}, {1, 2, 5}, {3, 1, 5}, {0, 3, 5}};
+    std::map<std::array<long long, 3>, uint32_t> ids;
+    auto vid = [&](const double* a, const double* b, const double* c, int i, int j) {
+        double p[3];
+        for (int d = 0; d < 3; ++d) p[d] = (a[d] * (n - i - j) + b[d] * i + c[d] * j) / n;
+        double l = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+        for (double& x : p) x /= l;
+        std::array<long long, 3> key{std::llround(p[0] * 1e9), std::llround(p[1] * 1e9), std::llround(p[2] * 1e9)};
+        auto it = ids.find(key);
+        if (it != ids.end()) return it->second;
+        uint32_t id = m.get_num_vertices();
+        m.add_vertex(p);
+        ids[key] = id;
+        return id;
+    };
+    for (auto& f : F) {
+        const double *a = P[f[0]], *b = P[f[1]], *c = P[f[2]];
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; i + j < n; ++j) {
+                m.add_triangle(vid(a, b, c, i, j), vid(a, b, c, i + 1, j), vid(a, b, c, i, j + 1));
+                if (i + j < n - 1)
+                    m.add_triangle(vid(a, b, c, i + 1, j), vid(a, b, c, i + 1, j + 1), vid(a, b, c, i, j + 1));
+            }
+    }
+    return m;
+}
+
+template <typename S, typename I>
+void cube_case()
+{
+    FastWindingNumber e(cube<S, I>());
+    CHECK(std::fabs(e.solid_angle({0.5f, 0.5f, 0.5f}) - 4 * kPi) < 1e-5);
+    CHECK(std::fabs(e.solid_angle({1.5f, 0.5f, 0.5f})) < 1e-5);
+    CHECK(e.is_inside({0.25f, 0.5f, 0.75f}));
+    CHECK(!e.is_inside({-0.1f, 0.5f, 0.5f}));
+}
+
+int main()
+{
+    std::vector<std::pair<std::string, std::function<void()>>> cases = {
+        {"cube float/uint32", cube_case<float, uint32_t>},
+        {"cube double/uint32", cube_case<double, uint32_t>},
+        {"cube float/uint64", cube_case<float, uint64_t>},
+        {"cube double/uint64", cube_case<double, uint64_t>},
+        {"octant triangle", [] {
+             SurfaceMesh<float, uint32_t> m;
+             m.add_vertex({1, 0, 0});
+             m.add_vertex({0, 1, 0});
+             m.add_vertex({0, 0, 1});
+             m.add_triangle(0, 1, 2);
+             FastWindingNumber e(m);
+             CHECK(std::fabs(e.solid_angle({0, 0, 0}) - kPi / 2) < 1e-6);
+         }},
+        {"rejects 2D and non-triangle meshes", [] {
+             bool threw = false;
+             try {
+                 SurfaceMesh<double, uint32_t> m(2);
+                 m.add_vertex({0, 0});
+                 m.add_vertex({1, 0});
+                 m.add_vertex({0, 1});
+                 m.add_triangle(0, 1, 2);
+                 FastWindingNumber e(m);
+             } catch (const lagrange::Error& ex) {
+                 threw = std::string(ex.what()).find("3D") != std::string::npos;
+             }
+             CHECK(threw);
+             threw = false;
+             try {
+                 SurfaceMesh<double, uint32_t> m;
+                 for (int i = 0; i < 4; ++i) m.add_vertex({double(i & 1), double(i >> 1), 0.0});
+                 m.add_quad(0, 1, 3, 2);
+                 FastWindingNumber e(m);
+             } catch (const lagrange::Error& ex) {
+                 threw = std::string(ex.what()).find("triangle") != std::string::npos;
+             }
+             CHECK(threw);
+         }},
+        {"move-only semantics", [] {
+             FastWindingNumber a(cube<float, uint32_t>());
+             FastWindingNumber b(std::move(a));
+             CHECK(b.is_inside({0.5f, 0.5f, 0.5f}));
+             FastWindingNumber c;
+             c = std::move(b);
+             CHECK(c.is_inside({0.5f, 0.5f, 0.5f}));
+             bool threw = false;
+             try {
+                 FastWindingNumber d;
+                 d.is_inside({0, 0, 0});
+             } catch (const lagrange::Error&) {
+                 threw = true;
+             }
+             CHECK(threw);
+         }},
+        {"sphere: batched equals per-point, inside/outside", [] {
+             FastWindingNumber e(sphere(24));
+             std::vector<std::array<float, 3>> pts;
+             for (int i = 0; i < 300; ++i) {
+                 float t = i / 299.0f;
+                 pts.push_back({-1.4f + 2.8f * t, 0.3f * std::sin(7.0f * t), 0.2f * std::cos(5.0f * t)});
+             }
+             std::vector<float> om(pts.size());
+             std::vector<char> in(pts.size());
+             e.solid_angle(pts.data(), pts.size(), om.data());
+             e.is_inside(pts.data(), pts.size(), reinterpret_cast<bool*>(in.data()));
+             for (size_t i = 0; i < pts.size(); ++i) {
+                 CHECK(e.solid_angle(pts[i]) == om[i]);
+                 CHECK(e.is_inside(pts[i]) == (in[i] != 0));
+                 double r = std::sqrt(pts[i][0] * pts[i][0] + pts[i][1] * pts[i][1] + pts[i][2] * pts[i][2]);
+                 if (r < 0.9) CHECK(in[i]);
+                 if (r > 1.1) CHECK(!in[i]);
+             }
+         }},
+        {"concurrent const queries", [] {
+             FastWindingNumber e(sphere(16));
+             std::vector<float> a(64), b(64);
+             auto work = [&](std::vector<float>& out) {
+                 for (int i = 0; i < 64; ++i) out[i] = e.solid_angle({0.02f * i - 0.6f, 0.1f, -0.2f});
+             };
+             std::thread t1(work, std::ref(a)), t2(work, std::ref(b));
+             t1.join();
+             t2.join();
+             CHECK(a == b);
+         }},
+    };
+    for (auto& c : cases) {
+        int before = g_fail;
+        try {
+            c.second();
+        } catch (const std::exception& ex) {
+            std::printf("  exception: %s\n", ex.what());
+            ++g_fail;
+        }
+        std::printf("%s %s\n", g_fail == before ? "PASS" : "FAIL", c.first.c_str());
+    }
+    std::printf("%s\n", g_fail ? "FAILED" : "ALL PASSED");
+    return g_fail ? 1 : 0;
+}

@@ -15,7 +15,7 @@
 //   k_bottom_up     arrival-counter walk to the roots: merge two children per node, FP32 for
 //                   small nodes and FP64 for nodes with >= fp64_moment_min triangles
 //   k_collapse_*    level by level, each wide node takes the binary node's children and
-//                   repeatedly opens the largest non-leaf slot until it has FWN_WIDTH slots
+//                   repeatedly opens the largest non-leaf slot until it has tree_width slots
 //   k_write_slots   FP64 records -> the 32-word FP32 slot format of fwn.h
 //
 // Several meshes are built together: their keys carry the mesh id above the Morton bits, so
@@ -383,7 +383,7 @@ __global__ __launch_bounds__(BT) void k_collapse_level(const int32_t* __restrict
                                                        const int32_t* __restrict__ right,
                                                        const int32_t* __restrict__ first,
                                                        const int32_t* __restrict__ last, int64_t n, int leaf_max,
-                                                       int32_t* __restrict__ wide_slot_bin,
+                                                       int width, int32_t* __restrict__ wide_slot_bin,
                                                        int32_t* __restrict__ wide_ns, int32_t* __restrict__ newcount)
 {
     int i = blockIdx.x * BT + threadIdx.x;
@@ -401,7 +401,7 @@ __global__ __launch_bounds__(BT) void k_collapse_level(const int32_t* __restrict
         sc[0] = node_count(sl[0], n, first, last);
         sc[1] = node_count(sl[1], n, first, last);
         ns = 2;
-        while (ns < kWidth) {
+        while (ns < width) {
             int best = -1, bc = leaf_max;
             for (int s = 0; s < ns; ++s)
                 if (sc[s] > bc) {
@@ -455,7 +455,7 @@ __global__ __launch_bounds__(BT) void k_collapse_assign(int32_t cnt, int32_t gba
     }
 }
 
-__global__ __launch_bounds__(BT) void k_write_slots(int64_t W, const int32_t* __restrict__ wide_slot_bin,
+__global__ __launch_bounds__(BT) void k_write_slots(int64_t W, int width, const int32_t* __restrict__ wide_slot_bin,
                                                     const int32_t* __restrict__ wide_ns,
                                                     const int32_t* __restrict__ child_wide,
                                                     const double* __restrict__ rec,
@@ -464,11 +464,12 @@ __global__ __launch_bounds__(BT) void k_write_slots(int64_t W, const int32_t* __
                                                     uint32_t* __restrict__ slots)
 {
     int64_t idx = (int64_t)blockIdx.x * BT + threadIdx.x;
-    if (idx >= W * kWidth) return;
-    int64_t g = idx / kWidth;
-    int s = (int)(idx % kWidth);
+    if (idx >= W * width) return;
+    int64_t g = idx / width;
+    int s = (int)(idx % width);
     uint32_t* w = slots + idx * kSlotWords;
-    int32_t u = wide_slot_bin[idx];
+    int64_t src = g * kWidth + s; // collapse scratch keeps kWidth entries per node
+    int32_t u = wide_slot_bin[src];
     if (s >= wide_ns[g] || u < 0) {
         for (int k = 0; k < kSlotWords; ++k) w[k] = 0u;
         return;
@@ -506,7 +507,7 @@ __global__ __launch_bounds__(BT) void k_write_slots(int64_t W, const int32_t* __
     // a leaf slot is any slot without a wide child: a single triangle or a bucket of at most
     // leaf_max triangles (an internal binary node); both cover a contiguous sorted range
     bool bin_leaf = u >= n - 1;
-    int32_t cw = child_wide[idx];
+    int32_t cw = child_wide[src];
     w[FWN_W_CHILD] = (uint32_t)cw;
     w[FWN_W_FIRST] = (uint32_t)(bin_leaf ? (int32_t)(u - (n - 1)) : first[u]);
     w[FWN_W_COUNT] = (uint32_t)(bin_leaf ? 1 : last[u] - first[u] + 1);
@@ -535,6 +536,8 @@ void build_engine(Engine& e, int32_t M, const float* V_host, const int64_t* v_of
     e.n_tris = nF;
     e.h_f_off.assign(f_off, f_off + M + 1);
     const int leaf_max = e.opt.leaf_max;
+    const int width = e.opt.tree_width;
+    e.width = width;
 
     hipEvent_t b0, b1;
     FWN_HIP(hipEventCreate(&b0));
@@ -589,7 +592,7 @@ void build_engine(Engine& e, int32_t M, const float* V_host, const int64_t* v_of
     for (int m = 0; m < M; ++m) e.h_roots[2 * m + 1] = 0;
 
     if (nF == 0) {
-        e.slots.alloc(sizeof(uint32_t) * kWidth * kSlotWords);
+        e.slots.alloc(sizeof(uint32_t) * width * kSlotWords);
         e.n_nodes = 0;
         e.max_depth = 0;
         FWN_HIP(hipMemcpyAsync(e.roots.p, e.h_roots.data(), sizeof(int32_t) * 2 * M, hipMemcpyHostToDevice, s));
@@ -684,7 +687,7 @@ void build_engine(Engine& e, int32_t M, const float* V_host, const int64_t* v_of
     while (cnt > 0) {
         hipLaunchKernelGGL(k_collapse_level, dim3(grid_blocks(cnt, BT)), dim3(BT), 0, s, lvl[cur].as<int32_t>(), cnt,
                            gbase, left.as<int32_t>(), right.as<int32_t>(), first.as<int32_t>(), last.as<int32_t>(),
-                           n, leaf_max, wsb.as<int32_t>(), wns.as<int32_t>(), newc.as<int32_t>());
+                           n, leaf_max, width, wsb.as<int32_t>(), wns.as<int32_t>(), newc.as<int32_t>());
         FWN_LAUNCH_CHECK("k_collapse_level");
         exclusive_scan_i32(newc.as<int32_t>(), offs.as<int32_t>(), cnt, scan_tmp, s);
         int32_t next_base = gbase + cnt;
@@ -704,8 +707,8 @@ void build_engine(Engine& e, int32_t M, const float* V_host, const int64_t* v_of
     const int64_t W = gbase;
     e.n_nodes = W;
     e.max_depth = depth;
-    e.slots.alloc(sizeof(uint32_t) * kWidth * kSlotWords * (size_t)std::max<int64_t>(W, 1));
-    hipLaunchKernelGGL(k_write_slots, dim3(grid_blocks(W * kWidth, BT)), dim3(BT), 0, s, W, wsb.as<int32_t>(),
+    e.slots.alloc(sizeof(uint32_t) * width * kSlotWords * (size_t)std::max<int64_t>(W, 1));
+    hipLaunchKernelGGL(k_write_slots, dim3(grid_blocks(W * width, BT)), dim3(BT), 0, s, W, width, wsb.as<int32_t>(),
                        wns.as<int32_t>(), cw.as<int32_t>(), rec.as<double>(), first.as<int32_t>(), last.as<int32_t>(),
                        n, e.slots.as<uint32_t>());
     FWN_LAUNCH_CHECK("k_write_slots");

@@ -55,6 +55,8 @@ void check_options(const fwn_options& o)
         fail(FWN_INVALID_ARGUMENT, "near_fp64_rel must be >= 0");
     if (o.leaf_max < 1 || o.leaf_max > 8) fail(FWN_INVALID_ARGUMENT, "leaf_max must be in [1, 8]");
     if (o.fp64_moment_min < 1) fail(FWN_INVALID_ARGUMENT, "fp64_moment_min must be >= 1");
+    if (o.tree_width != 2 && o.tree_width != 4 && o.tree_width != 8)
+        fail(FWN_INVALID_ARGUMENT, "tree_width must be 2, 4 or 8");
 }
 
 Engine* make_engine(const fwn_options* opt)
@@ -182,6 +184,7 @@ int fwn_options_default(fwn_options* opt)
     opt->fp64_moment_min = 256;
     opt->device = 0;
     opt->waves_per_block = 0;
+    opt->tree_width = 4;
     return FWN_OK;
 }
 
@@ -264,7 +267,7 @@ int fwn_tree_info_get(const fwn_engine* h, fwn_tree_info* info)
         info->num_meshes = e->n_meshes;
         info->num_nodes = e->n_nodes;
         info->num_triangles = e->n_tris;
-        info->width = fwn::kWidth;
+        info->width = e->width;
         info->max_depth = e->max_depth;
         info->build_ms = e->build_ms;
     });
@@ -277,7 +280,7 @@ int fwn_export_tree(const fwn_engine* h, uint32_t* slots, uint32_t* tris, int32_
         std::lock_guard<std::mutex> lk(e->mu);
         fwn::DeviceGuard g(e->device);
         if (slots && e->n_nodes)
-            FWN_HIP(hipMemcpy(slots, e->slots.p, sizeof(uint32_t) * fwn::kWidth * fwn::kSlotWords * e->n_nodes,
+            FWN_HIP(hipMemcpy(slots, e->slots.p, sizeof(uint32_t) * e->width * fwn::kSlotWords * e->n_nodes,
                               hipMemcpyDeviceToHost));
         if (tris && e->n_tris)
             FWN_HIP(hipMemcpy(tris, e->tris.p, sizeof(uint32_t) * fwn::kTriWords * e->n_tris, hipMemcpyDeviceToHost));

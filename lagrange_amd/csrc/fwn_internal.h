@@ -31,11 +31,12 @@ void hip_check(hipError_t e, const char* what, const char* file, int line);
 // ---------------------------------------------------------------------------------------
 // constants shared by host and device code
 // ---------------------------------------------------------------------------------------
-constexpr int kWidth = FWN_WIDTH;          // slots per wide node
+constexpr int kWidth = FWN_WIDTH;          // maximum slots per node
 constexpr int kSlotWords = FWN_SLOT_WORDS; // 32-bit words per slot
 constexpr int kTriWords = FWN_TRI_WORDS;   // 32-bit words per stored triangle
 constexpr int kRecWords = 32;              // doubles per binary-node moment record
 constexpr int kStack = 192;                // per-wave traversal stack entries
+constexpr double kFixScale = 1099511627776.0; // 2^40: fixed-point scale of FP64 near-field sums
 constexpr int kWave = 64;
 
 // binary-node moment record layout (doubles)
@@ -62,13 +63,14 @@ struct Engine {
     fwn_options opt{};
 
     int32_t n_meshes = 0;
+    int32_t width = 4;         // slots per node
     int64_t n_tris = 0;
     int64_t n_nodes = 0;
     int32_t max_depth = 0;
     double build_ms = 0.0;
 
     // tree (resident)
-    DBuf slots;      // n_nodes * kWidth * kSlotWords u32
+    DBuf slots;      // n_nodes * width * kSlotWords u32
     DBuf tris;       // n_tris * kTriWords u32
     DBuf roots;      // 2 int32 per mesh: wide root, occupied slots
     DBuf mesh_info;  // per mesh: 8 floats: qbox lo[3], qbox inv[3], tau, pad
@@ -79,6 +81,11 @@ struct Engine {
     // query scratch (grown on demand)
     DBuf q_keys[2], q_vals[2], q_hist, q_rowsum, q_in, q_om, q_in8, q_off, q_boff, q_stats, q_flag;
     DBuf q_bf;
+    // deferred FP64 near-field work: (sorted point, first triangle, count) items and per-point
+    // fixed-point accumulators (zero between calls)
+    DBuf q_items, q_acc;
+    int64_t item_cap = 0;
+    int64_t acc_cap = 0;
 
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double last_traverse_ms = 0.0;
@@ -86,12 +93,6 @@ struct Engine {
 };
 
 // sorting / scanning primitives (fwn_sort.hip)
-struct SortScratch {
-    DBuf* keys2;
-    DBuf* vals2;
-    DBuf* hist;
-    DBuf* rowsum;
-};
 // Stable LSD radix sort of (keys, vals) pairs by key bits [0, nbits).  Result ends in keys/vals
 // (swapping with the scratch buffers as needed; the pointers are updated).
 void radix_sort_pairs(uint64_t*& keys, uint32_t*& vals, uint64_t*& keys_alt, uint32_t*& vals_alt, int64_t n,

@@ -163,10 +163,11 @@ __device__ __forceinline__ void stack_pop(const WaveStack& S, int sp, uint32_t& 
 
 template <bool STATS, int WAVES>
 __global__ __launch_bounds__(WAVES * 64) void k_traverse(
-    const uint32_t* __restrict__ slots, const uint32_t* __restrict__ tris, const int32_t* __restrict__ roots,
-    const float* __restrict__ minfo, const float* __restrict__ Q, const uint32_t* __restrict__ perm,
-    const int64_t* __restrict__ q_off, const int64_t* __restrict__ b_off, int M, int64_t n_batches, float beta2,
-    float* __restrict__ omega, uint8_t* __restrict__ inside, int32_t* __restrict__ stats,
+    const uint32_t* __restrict__ slots, int width, const uint32_t* __restrict__ tris,
+    const int32_t* __restrict__ roots, const float* __restrict__ minfo, const float* __restrict__ Q,
+    const uint32_t* __restrict__ perm, const int64_t* __restrict__ q_off, const int64_t* __restrict__ b_off, int M,
+    int64_t n_batches, float beta2, float* __restrict__ omega, uint8_t* __restrict__ inside,
+    int32_t* __restrict__ stats, uint32_t* __restrict__ items, uint32_t* __restrict__ n_items, uint32_t item_cap,
     int32_t* __restrict__ flag)
 {
     const int lane = threadIdx.x & 63;
@@ -178,13 +179,16 @@ __global__ __launch_bounds__(WAVES * 64) void k_traverse(
     const int64_t cnt64 = q_off[m + 1] - start;
     const int cnt = cnt64 < 64 ? (int)cnt64 : 64;
     const bool valid = lane < cnt;
-    const uint32_t orig = valid ? (perm ? perm[start + lane] : (uint32_t)(start + lane)) : 0u;
+    const uint32_t jpos = (uint32_t)(start + lane); // sorted position of this lane's point
+    const uint32_t orig = valid ? (perm ? perm[jpos] : jpos) : 0u;
     const float qx = valid ? Q[3 * (int64_t)orig] : 0.0f;
     const float qy = valid ? Q[3 * (int64_t)orig + 1] : 0.0f;
     const float qz = valid ? Q[3 * (int64_t)orig + 2] : 0.0f;
     const int32_t root = roots[2 * m];
     const int32_t root_ns = roots[2 * m + 1];
     const float tau = minfo[8 * m + 6];
+    const float tau2 = tau * tau;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
     float om = 0.0f;
     int32_t n_test = 0, n_far = 0, n_t32 = 0, n_t64 = 0;
@@ -199,7 +203,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_traverse(
             uint64_t mk;
             stack_pop(S, sp, e, mk);
             const bool act = (mk >> lane) & 1ull;
-            const uint32_t* nb = slots + (size_t)(e >> 4) * (kWidth * kSlotWords);
+            const uint32_t* nb = slots + (size_t)(e >> 4) * (size_t)width * kSlotWords;
             const int ns = (int)(e & 15u);
             for (int s = 0; s < ns; ++s) {
                 const float* w = reinterpret_cast<const float*>(nb + s * kSlotWords);
@@ -219,18 +223,36 @@ __global__ __launch_bounds__(WAVES * 64) void k_traverse(
                     const int32_t first = __float_as_int(w[FWN_W_FIRST]);
                     const int32_t tcnt = __float_as_int(w[FWN_W_COUNT]);
                     const bool nl = (nearm >> lane) & 1ull;
+                    // FP64 for points within tau of the leaf's bounding sphere and, for a single
+                    // triangle, within tau of its plane (p lies on that plane)
                     const float rr = sqrtf(r2) + tau;
-                    const bool use64 = nl && tau > 0.0f && d2 < rr * rr;
-                    for (int32_t t = first; t < first + tcnt; ++t) {
-                        const float* tw = reinterpret_cast<const float*>(tris + (size_t)t * kTriWords);
-                        if (nl) {
-                            if (use64) {
-                                om += (float)tri_f64(tw, qx, qy, qz);
-                                if (STATS) ++n_t64;
-                            } else {
+                    const float nd = fmaf(dx, w[FWN_W_NX], fmaf(dy, w[FWN_W_NY], dz * w[FWN_W_NZ]));
+                    const float nn = fmaf(w[FWN_W_NX], w[FWN_W_NX], fmaf(w[FWN_W_NY], w[FWN_W_NY], w[FWN_W_NZ] * w[FWN_W_NZ]));
+                    const bool use64 = nl && tau > 0.0f && d2 < rr * rr && (tcnt > 1 || nd * nd < tau2 * nn);
+                    const bool use32 = nl && !use64;
+                    if (__ballot(use32) != 0ull) {
+                        for (int32_t t = first; t < first + tcnt; ++t) {
+                            const float* tw = reinterpret_cast<const float*>(tris + (size_t)t * kTriWords);
+                            if (use32) {
                                 om += tri_f32(tw, qx, qy, qz);
                                 if (STATS) ++n_t32;
                             }
+                        }
+                    }
+                    const uint64_t m64 = __ballot(use64);
+                    if (m64 != 0ull) {
+                        // deferred: (point, leaf) items, evaluated in FP64 by k_near64
+                        uint32_t base = 0u;
+                        if (lane == 0) base = atomicAdd(n_items, (uint32_t)__popcll(m64));
+                        base = __builtin_amdgcn_readfirstlane(base);
+                        if (use64) {
+                            const uint32_t k = base + (uint32_t)__popcll(m64 & lt_mask);
+                            if (k < item_cap) {
+                                items[3 * (size_t)k] = jpos;
+                                items[3 * (size_t)k + 1] = (uint32_t)first;
+                                items[3 * (size_t)k + 2] = (uint32_t)tcnt;
+                            }
+                            if (STATS) n_t64 += tcnt;
                         }
                     }
                 } else if (sp < kStack) {
@@ -256,6 +278,53 @@ __global__ __launch_bounds__(WAVES * 64) void k_traverse(
     }
 }
 
+// FP64 solid angles of the deferred (point, leaf) items, summed per point in 2^-40 fixed
+// point: integer addition is associative, so the per-point sum does not depend on the order
+// in which items are processed.
+__global__ __launch_bounds__(256) void k_near64(const uint32_t* __restrict__ items,
+                                                const uint32_t* __restrict__ n_items, uint32_t item_cap,
+                                                const uint32_t* __restrict__ tris, const float* __restrict__ Q,
+                                                const uint32_t* __restrict__ perm,
+                                                unsigned long long* __restrict__ acc)
+{
+    const uint32_t n = min(*n_items, item_cap);
+    for (uint32_t k = blockIdx.x * 256u + threadIdx.x; k < n; k += gridDim.x * 256u) {
+        const uint32_t j = items[3 * (size_t)k], first = items[3 * (size_t)k + 1], cnt = items[3 * (size_t)k + 2];
+        const uint32_t orig = perm ? perm[j] : j;
+        const float qx = Q[3 * (size_t)orig], qy = Q[3 * (size_t)orig + 1], qz = Q[3 * (size_t)orig + 2];
+        double s = 0.0;
+        for (uint32_t t = first; t < first + cnt; ++t)
+            s += tri_f64(reinterpret_cast<const float*>(tris + (size_t)t * kTriWords), qx, qy, qz);
+        atomicAdd(&acc[j], (unsigned long long)__double2ll_rn(s * kFixScale));
+    }
+}
+
+// omega = fp32 sum + FP64 near-field sum, once per point: the item that takes the accumulator
+// (atomic exchange with 0) does the update, which also leaves the accumulator zeroed.
+__global__ __launch_bounds__(256) void k_near64_combine(const uint32_t* __restrict__ items,
+                                                        const uint32_t* __restrict__ n_items, uint32_t item_cap,
+                                                        const uint32_t* __restrict__ perm,
+                                                        unsigned long long* __restrict__ acc,
+                                                        float* __restrict__ omega, uint8_t* __restrict__ inside)
+{
+    const uint32_t n = min(*n_items, item_cap);
+    for (uint32_t k = blockIdx.x * 256u + threadIdx.x; k < n; k += gridDim.x * 256u) {
+        const uint32_t j = items[3 * (size_t)k];
+        const long long v = (long long)atomicExch(&acc[j], 0ull);
+        if (v == 0) continue;
+        const uint32_t orig = perm ? perm[j] : j;
+        const float om = (float)((double)omega[orig] + (double)v * (1.0 / kFixScale));
+        omega[orig] = om;
+        if (inside) inside[orig] = ((double)om / (4.0 * 3.14159265358979323846) > 0.5) ? 1 : 0;
+    }
+}
+
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
 __global__ __launch_bounds__(256) void k_bruteforce(const uint32_t* __restrict__ tris, int64_t first, int64_t cnt,
                                                     const float* __restrict__ Q, int64_t nQ,
                                                     double* __restrict__ out)
@@ -269,7 +338,11 @@ __global__ __launch_bounds__(256) void k_bruteforce(const uint32_t* __restrict__
     if (valid) out[i] = s;
 }
 
-__global__ void k_zero_i32(int32_t* p) { *p = 0; }
+__global__ void k_zero2(int32_t* a, uint32_t* b)
+{
+    *a = 0;
+    *b = 0u;
+}
 
 } // namespace
 
@@ -293,7 +366,7 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
     if (nQ < 0) fail(FWN_INVALID_ARGUMENT, "negative point count");
     if (nQ == 0) return;
     if (nQ > (int64_t)UINT32_MAX / 2) fail(FWN_LIMIT_EXCEEDED, "more than 2^31 points in one call");
-    if (1 + (kWidth - 1) * (int64_t)e.max_depth > kStack)
+    if (1 + (int64_t)(e.width - 1) * e.max_depth > kStack)
         fail(FWN_LIMIT_EXCEEDED, "tree deeper than the traversal stack supports");
 
     std::vector<int64_t> boff(M + 1, 0);
@@ -321,7 +394,7 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
     }
     e.q_off.reserve(sizeof(int64_t) * (M + 1));
     e.q_boff.reserve(sizeof(int64_t) * (M + 1));
-    e.q_flag.reserve(sizeof(int32_t));
+    e.q_flag.reserve(2 * sizeof(int32_t));
     FWN_HIP(hipMemcpyAsync(e.q_off.p, qoff.data(), sizeof(int64_t) * (M + 1), hipMemcpyHostToDevice, s));
     FWN_HIP(hipMemcpyAsync(e.q_boff.p, boff.data(), sizeof(int64_t) * (M + 1), hipMemcpyHostToDevice, s));
 
@@ -331,9 +404,20 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
     }
     if (!e.ev[0])
         for (int k = 0; k < 4; ++k) FWN_HIP(hipEventCreate(&e.ev[k]));
+    // per-point FP64 accumulators (sorted positions), kept zero between calls
+    if (e.acc_cap < nQ) {
+        e.q_acc.alloc(sizeof(unsigned long long) * nQ);
+        e.acc_cap = nQ;
+        hipLaunchKernelGGL(k_fill_u64, dim3(grid_blocks(nQ, 256)), dim3(256), 0, s,
+                           e.q_acc.as<unsigned long long>(), nQ);
+        FWN_LAUNCH_CHECK("k_fill_u64");
+    }
+    if (e.item_cap < std::max<int64_t>(nQ, 1 << 20)) {
+        e.item_cap = std::min<int64_t>(std::max<int64_t>(nQ, 1 << 20), (int64_t)UINT32_MAX / 4);
+        e.q_items.alloc(sizeof(uint32_t) * 3 * (size_t)e.item_cap);
+    }
 
     FWN_HIP(hipEventRecord(e.ev[0], s));
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, s, e.q_flag.as<int32_t>());
     uint64_t* kp = e.q_keys[0].as<uint64_t>();
     uint32_t* vp = e.q_vals[0].as<uint32_t>();
     uint64_t* kp2 = e.q_keys[1].as<uint64_t>();
@@ -349,21 +433,42 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
     const int64_t nblk = (nb + WAVES - 1) / WAVES;
     if (nblk > INT32_MAX) fail(FWN_LIMIT_EXCEEDED, "too many point batches");
     const float beta2 = e.opt.accuracy_scale * e.opt.accuracy_scale;
-    FWN_HIP(hipEventRecord(e.ev[1], s));
-    if (stats)
-        hipLaunchKernelGGL((k_traverse<true, WAVES>), dim3((unsigned)nblk), dim3(WAVES * 64), 0, s,
-                           e.slots.as<uint32_t>(), e.tris.as<uint32_t>(), e.roots.as<int32_t>(),
-                           e.mesh_info.as<float>(), dQ, vp, e.q_off.as<int64_t>(), e.q_boff.as<int64_t>(), M, nb,
-                           beta2, dOm, dIn, dSt, e.q_flag.as<int32_t>());
-    else
-        hipLaunchKernelGGL((k_traverse<false, WAVES>), dim3((unsigned)nblk), dim3(WAVES * 64), 0, s,
-                           e.slots.as<uint32_t>(), e.tris.as<uint32_t>(), e.roots.as<int32_t>(),
-                           e.mesh_info.as<float>(), dQ, vp, e.q_off.as<int64_t>(), e.q_boff.as<int64_t>(), M, nb,
-                           beta2, dOm, dIn, dSt, e.q_flag.as<int32_t>());
-    FWN_LAUNCH_CHECK("k_traverse");
-    FWN_HIP(hipEventRecord(e.ev[2], s));
-    int32_t h_flag = 0;
-    FWN_HIP(hipMemcpyAsync(&h_flag, e.q_flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    int32_t* dflag = e.q_flag.as<int32_t>();
+    uint32_t* dnitems = reinterpret_cast<uint32_t*>(dflag + 1);
+    for (int attempt = 0;; ++attempt) {
+        hipLaunchKernelGGL(k_zero2, dim3(1), dim3(1), 0, s, dflag, dnitems);
+        FWN_HIP(hipEventRecord(e.ev[1], s));
+        const uint32_t cap = (uint32_t)e.item_cap;
+        if (stats)
+            hipLaunchKernelGGL((k_traverse<true, WAVES>), dim3((unsigned)nblk), dim3(WAVES * 64), 0, s,
+                               e.slots.as<uint32_t>(), e.width, e.tris.as<uint32_t>(), e.roots.as<int32_t>(),
+                               e.mesh_info.as<float>(), dQ, vp, e.q_off.as<int64_t>(), e.q_boff.as<int64_t>(), M, nb,
+                               beta2, dOm, dIn, dSt, e.q_items.as<uint32_t>(), dnitems, cap, dflag);
+        else
+            hipLaunchKernelGGL((k_traverse<false, WAVES>), dim3((unsigned)nblk), dim3(WAVES * 64), 0, s,
+                               e.slots.as<uint32_t>(), e.width, e.tris.as<uint32_t>(), e.roots.as<int32_t>(),
+                               e.mesh_info.as<float>(), dQ, vp, e.q_off.as<int64_t>(), e.q_boff.as<int64_t>(), M, nb,
+                               beta2, dOm, dIn, dSt, e.q_items.as<uint32_t>(), dnitems, cap, dflag);
+        FWN_LAUNCH_CHECK("k_traverse");
+        FWN_HIP(hipEventRecord(e.ev[2], s));
+        hipLaunchKernelGGL(k_near64, dim3(2048), dim3(256), 0, s, e.q_items.as<uint32_t>(), dnitems, cap,
+                           e.tris.as<uint32_t>(), dQ, vp, e.q_acc.as<unsigned long long>());
+        FWN_LAUNCH_CHECK("k_near64");
+        hipLaunchKernelGGL(k_near64_combine, dim3(2048), dim3(256), 0, s, e.q_items.as<uint32_t>(), dnitems, cap, vp,
+                           e.q_acc.as<unsigned long long>(), dOm, dIn);
+        FWN_LAUNCH_CHECK("k_near64_combine");
+        FWN_HIP(hipEventRecord(e.ev[3], s));
+        int32_t h_fl[2] = {0, 0};
+        FWN_HIP(hipMemcpyAsync(h_fl, dflag, sizeof(h_fl), hipMemcpyDeviceToHost, s));
+        FWN_HIP(hipStreamSynchronize(s));
+        if (h_fl[0]) fail(FWN_LIMIT_EXCEEDED, "traversal stack overflow");
+        const uint32_t need = (uint32_t)h_fl[1];
+        if ((int64_t)need <= e.item_cap) break;
+        if (attempt > 0 || (int64_t)need > (int64_t)UINT32_MAX / 4)
+            fail(FWN_LIMIT_EXCEEDED, "FP64 near-field work list too large");
+        e.item_cap = (int64_t)need + need / 4;
+        e.q_items.alloc(sizeof(uint32_t) * 3 * (size_t)e.item_cap);
+    }
     if (!dev) {
         FWN_HIP(hipMemcpyAsync(omega, dOm, sizeof(float) * nQ, hipMemcpyDeviceToHost, s));
         if (inside) FWN_HIP(hipMemcpyAsync(inside, dIn, nQ, hipMemcpyDeviceToHost, s));
@@ -372,10 +477,9 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
     FWN_HIP(hipStreamSynchronize(s));
     float t_tr = 0.0f, t_all = 0.0f;
     FWN_HIP(hipEventElapsedTime(&t_tr, e.ev[1], e.ev[2]));
-    FWN_HIP(hipEventElapsedTime(&t_all, e.ev[0], e.ev[2]));
+    FWN_HIP(hipEventElapsedTime(&t_all, e.ev[0], e.ev[3]));
     e.last_traverse_ms = t_tr;
     e.last_total_ms = t_all;
-    if (h_flag) fail(FWN_LIMIT_EXCEEDED, "traversal stack overflow");
 }
 
 void run_bruteforce(Engine& e, const float* Q, int64_t nQ, double* out, bool dev)

@@ -79,9 +79,9 @@ class _EngineBase:
         return {k: getattr(info, k) for k, _ in L.TreeInfo._fields_}
 
     def export_tree(self):
-        """(slots u32 [nodes*8*32], tris u32 [F*12], roots int32 [M,2], tau float32 [M])."""
+        """(slots u32 [nodes*width*32], tris u32 [F*12], roots int32 [M,2], tau float32 [M])."""
         info = self.tree_info()
-        slots = np.zeros(max(info["num_nodes"], 1) * L.WIDTH * L.SLOT_WORDS, np.uint32)
+        slots = np.zeros(max(info["num_nodes"], 1) * info["width"] * L.SLOT_WORDS, np.uint32)
         tris = np.zeros(max(info["num_triangles"], 1) * L.TRI_WORDS, np.uint32)
         roots = np.zeros((info["num_meshes"], 2), np.int32)
         tau = np.zeros(info["num_meshes"], np.float32)

@@ -294,7 +294,8 @@ static void slot_from_moments(const double* m, uint32_t* w)
 /* ------------------------------------------------------------------------------------ */
 typedef struct restated_tree {
     int64_t nnodes, cap;
-    uint32_t* slots; /* nnodes * 4 * 32 words */
+    int32_t width;   /* slots per node: 4 restates the reference; 2 and 8 are for study */
+    uint32_t* slots; /* nnodes * width * 32 words */
     uint32_t* tris;  /* nF * 12 words, in tree order */
     int64_t nF;
     int32_t root_ns;
@@ -403,36 +404,42 @@ static int64_t rnode_alloc(restated_tree* T)
 {
     if (T->nnodes == T->cap) {
         T->cap = T->cap ? 2 * T->cap : 1024;
-        T->slots = (uint32_t*)realloc(T->slots, sizeof(uint32_t) * 4 * FWN_SLOT_WORDS * (size_t)T->cap);
+        T->slots = (uint32_t*)realloc(T->slots, sizeof(uint32_t) * T->width * FWN_SLOT_WORDS * (size_t)T->cap);
     }
     int64_t id = T->nnodes++;
-    memset(T->slots + 4 * FWN_SLOT_WORDS * id, 0, sizeof(uint32_t) * 4 * FWN_SLOT_WORDS);
+    memset(T->slots + T->width * FWN_SLOT_WORDS * id, 0, sizeof(uint32_t) * T->width * FWN_SLOT_WORDS);
     return id;
 }
 
 /* builds the node for items[lo,hi) (n >= 2), fills its slots, returns moment record */
 static int rbuild_node(rbuild* B, int64_t lo, int64_t hi, int64_t node, double* mout, int depth)
 {
-    int64_t bounds[5];
-    int ng;
-    int64_t mid = split2(B, lo, hi);
-    int64_t parts[3] = {lo, mid, hi};
-    ng = 0;
+    /* split the range into up to `width` parts: log2(width) rounds of binary box-area splits,
+     * each round splitting every part that holds two or more triangles (width 4: two rounds,
+     * the reference's 4-ary layout) */
+    const int width = B->T->width;
+    int64_t bounds[9];
+    int ng = 1;
     bounds[0] = lo;
-    for (int h = 0; h < 2; ++h) {
-        int64_t a = parts[h], b = parts[h + 1];
-        if (b - a >= 2) {
-            int64_t m2 = split2(B, a, b);
-            bounds[++ng] = m2;
+    bounds[1] = hi;
+    for (int round = 1; round < width; round <<= 1) {
+        int64_t nb[9];
+        int nn = 0;
+        nb[0] = lo;
+        for (int g = 0; g < ng; ++g) {
+            int64_t a = bounds[g], b = bounds[g + 1];
+            if (b - a >= 2) nb[++nn] = split2(B, a, b);
+            nb[++nn] = b;
         }
-        bounds[++ng] = b;
+        ng = nn;
+        memcpy(bounds, nb, sizeof(int64_t) * (nn + 1));
     }
-    double rec[4][MSIZE];
-    const double* chp[4];
+    double rec[8][MSIZE];
+    const double* chp[8];
     int maxd = depth;
     for (int s = 0; s < ng; ++s) {
         int64_t a = bounds[s], b = bounds[s + 1];
-        uint32_t* w = B->T->slots + (4 * node + s) * FWN_SLOT_WORDS;
+        uint32_t* w = B->T->slots + ((int64_t)width * node + s) * FWN_SLOT_WORDS;
         int32_t child = -1, child_ns = 0;
         int64_t first = B->tri_out;
         if (b - a == 1) {
@@ -455,10 +462,10 @@ static int rbuild_node(rbuild* B, int64_t lo, int64_t hi, int64_t node, double* 
             int d2 = rbuild_node(B, a, b, cn, rec[s], depth + 1);
             if (d2 > maxd) maxd = d2;
             /* the child's slot count */
-            w = B->T->slots + (4 * node + s) * FWN_SLOT_WORDS; /* slots may have moved */
-            uint32_t* cw = B->T->slots + 4 * cn * FWN_SLOT_WORDS;
+            w = B->T->slots + ((int64_t)width * node + s) * FWN_SLOT_WORDS; /* slots may have moved */
+            uint32_t* cw = B->T->slots + (int64_t)width * cn * FWN_SLOT_WORDS;
             child_ns = 0;
-            for (int k = 0; k < 4; ++k)
+            for (int k = 0; k < width; ++k)
                 if (cw[k * FWN_SLOT_WORDS + FWN_W_COUNT] != 0) child_ns++;
         }
         slot_from_moments(rec[s], w);
@@ -470,10 +477,11 @@ static int rbuild_node(rbuild* B, int64_t lo, int64_t hi, int64_t node, double* 
     return maxd;
 }
 
-API restated_tree* restated_build(const float* V, int64_t nV, const int32_t* F, int64_t nF)
+API restated_tree* restated_build_w(const float* V, int64_t nV, const int32_t* F, int64_t nF, int32_t width)
 {
     (void)nV;
     restated_tree* T = (restated_tree*)calloc(1, sizeof(restated_tree));
+    T->width = (width == 2 || width == 8) ? width : 4;
     T->nF = nF;
     T->tris = (uint32_t*)calloc((size_t)(nF > 0 ? nF : 1) * FWN_TRI_WORDS, sizeof(uint32_t));
     rbuild B;
@@ -515,13 +523,18 @@ API restated_tree* restated_build(const float* V, int64_t nV, const int32_t* F, 
         T->depth = 1;
     }
     T->root_ns = 0;
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < T->width; ++k)
         if (T->slots[k * FWN_SLOT_WORDS + FWN_W_COUNT] != 0) T->root_ns++;
     free(B.cen);
     free(B.blo);
     free(B.bhi);
     free(B.items);
     return T;
+}
+
+API restated_tree* restated_build(const float* V, int64_t nV, const int32_t* F, int64_t nF)
+{
+    return restated_build_w(V, nV, F, nF, 4);
 }
 
 API void restated_free(restated_tree* T)
@@ -603,8 +616,11 @@ static inline float far_field(const float* w, float dx, float dy, float dz, floa
 typedef struct { int32_t node; int32_t ns; } tstack;
 
 /* One point over a slot tree of the given width.  Children are pushed in slot order and popped
- * last-in-first-out, which is the order the GPU kernel sums a point's terms in.  tau > 0:
- * triangles of a near leaf whose bounding sphere (sqrt(r2)+tau) contains q use FP64. */
+ * last-in-first-out, which is the order the GPU kernel sums a point's FP32 terms in.  With
+ * tau > 0, a near leaf's triangles are taken in FP64 when q lies within tau of the leaf's
+ * bounding sphere (sqrt(r2) + tau) and, for a single-triangle leaf, within tau of its plane;
+ * each such leaf's FP64 sum is rounded to 2^-40 fixed point and the fixed-point total is added
+ * to the FP32 sum once, at the end (the engine's deferred near-field pass does the same). */
 static float eval_point(const uint32_t* slots, int width, int32_t root, int32_t root_ns,
                         const uint32_t* tris, float beta2, float tau, float qx, float qy, float qz,
                         int32_t* st)
@@ -615,6 +631,8 @@ static float eval_point(const uint32_t* slots, int width, int32_t root, int32_t 
     stack[sp].ns = root_ns;
     sp++;
     float om = 0.0f;
+    long long acc = 0;
+    const float tau2 = tau * tau;
     int32_t nt = 0, nf = 0, n32 = 0, n64 = 0;
     while (sp > 0) {
         --sp;
@@ -631,14 +649,18 @@ static float eval_point(const uint32_t* slots, int width, int32_t root, int32_t 
             } else if (wi[FWN_W_CHILD] < 0) {
                 int32_t first = wi[FWN_W_FIRST], cnt = wi[FWN_W_COUNT];
                 float rr = sqrtf(w[FWN_W_R2]) + tau;
-                int use64 = tau > 0.0f && d2 < rr * rr;
-                for (int32_t t = first; t < first + cnt; ++t) {
-                    const float* tw = (const float*)(tris + (int64_t)t * FWN_TRI_WORDS);
-                    if (use64) {
-                        om += (float)tri_f64_words(tw, qx, qy, qz);
-                        n64++;
-                    } else {
-                        om += tri_f32(tw, qx, qy, qz);
+                float nd = fmaf(dx, w[FWN_W_NX], fmaf(dy, w[FWN_W_NY], dz * w[FWN_W_NZ]));
+                float nn = fmaf(w[FWN_W_NX], w[FWN_W_NX], fmaf(w[FWN_W_NY], w[FWN_W_NY], w[FWN_W_NZ] * w[FWN_W_NZ]));
+                int use64 = tau > 0.0f && d2 < rr * rr && (cnt > 1 || nd * nd < tau2 * nn);
+                if (use64) {
+                    double s64 = 0.0;
+                    for (int32_t t = first; t < first + cnt; ++t)
+                        s64 += tri_f64_words((const float*)(tris + (int64_t)t * FWN_TRI_WORDS), qx, qy, qz);
+                    acc += llrint(s64 * 1099511627776.0);
+                    n64 += cnt;
+                } else {
+                    for (int32_t t = first; t < first + cnt; ++t) {
+                        om += tri_f32((const float*)(tris + (int64_t)t * FWN_TRI_WORDS), qx, qy, qz);
                         n32++;
                     }
                 }
@@ -650,6 +672,7 @@ static float eval_point(const uint32_t* slots, int width, int32_t root, int32_t 
             }
         }
     }
+    if (acc != 0) om = (float)((double)om + (double)acc * (1.0 / 1099511627776.0));
     if (st) {
         st[0] = nt;
         st[1] = nf;

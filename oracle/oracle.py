@@ -53,6 +53,8 @@ def lib():
         L.oracle_moments.argtypes = [FP, I32P, I64P, C.c_int64, DP]
         L.restated_build.restype = C.c_void_p
         L.restated_build.argtypes = [FP, C.c_int64, I32P, C.c_int64]
+        L.restated_build_w.restype = C.c_void_p
+        L.restated_build_w.argtypes = [FP, C.c_int64, I32P, C.c_int64, C.c_int32]
         L.restated_free.argtypes = [C.c_void_p]
         L.restated_num_nodes.restype = C.c_int64
         L.restated_num_nodes.argtypes = [C.c_void_p]
@@ -148,22 +150,25 @@ def tree_eval(slots, width, root, root_ns, tris, Q, beta=2.0, tau=0.0, stats=Fal
 class Restated:
     """CPU restatement of the reference engine (FP32, order 2, beta = 2 by default)."""
 
-    def __init__(self, V, F):
+    def __init__(self, V, F, width: int = WIDTH_RESTATED):
+        """width 4 restates the reference's 4-ary tree; 2 and 8 exist for comparison."""
         self.V, self.F = _f32(V), _i32(F)
+        self.width = int(width)
         L = lib()
-        self._h = L.restated_build(_p(self.V, FP), self.V.shape[0], _p(self.F, I32P), self.F.shape[0])
+        self._h = L.restated_build_w(_p(self.V, FP), self.V.shape[0], _p(self.F, I32P), self.F.shape[0], self.width)
         n = L.restated_num_nodes(self._h)
         self.num_nodes = int(n)
         self.root_ns = int(L.restated_root_ns(self._h))
         self.depth = int(L.restated_depth(self._h))
-        self.slots = np.ctypeslib.as_array(L.restated_slots(self._h), shape=(max(n, 1) * 4 * SLOT_WORDS,)).copy()
+        self.slots = np.ctypeslib.as_array(L.restated_slots(self._h),
+                                           shape=(max(n, 1) * self.width * SLOT_WORDS,)).copy()
         self.tris = np.ctypeslib.as_array(L.restated_tris(self._h),
                                           shape=(max(self.F.shape[0], 1) * TRI_WORDS,)).copy()
         L.restated_free(self._h)
         self._h = None
 
     def solid_angle(self, Q, beta: float = 2.0, stats: bool = False, nthreads: int = 0):
-        return tree_eval(self.slots, WIDTH_RESTATED, 0, self.root_ns, self.tris, Q, beta, 0.0, stats, nthreads)
+        return tree_eval(self.slots, self.width, 0, self.root_ns, self.tris, Q, beta, 0.0, stats, nthreads)
 
     def is_inside(self, Q, beta: float = 2.0):
         return inside_from_omega(self.solid_angle(Q, beta))

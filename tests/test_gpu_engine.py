@@ -43,17 +43,18 @@ def test_sorted_triangle_order_is_morton_order(oracle, ico5):
     np.testing.assert_array_equal(got, want)
 
 
-@pytest.mark.parametrize("leaf_max", [1, 4])
-def test_tree_structure_and_moments(oracle, leaf_max):
+@pytest.mark.parametrize("leaf_max,width", [(1, 4), (4, 4), (1, 2), (2, 8)])
+def test_tree_structure_and_moments(oracle, leaf_max, width):
     m = W.displaced_sphere(12, 5, delete_fraction=0.05)
     V, F = m.f32()
-    e = FastWindingNumber((V, F), leaf_max=leaf_max)
+    e = FastWindingNumber((V, F), leaf_max=leaf_max, tree_width=width)
     info = e.tree_info()
+    assert info["width"] == width
     slots, tris, roots, tau = e.export_tree()
     fl, it = slot_view(slots)
     tri_ids = tris.reshape(-1, 12)[:, 9].astype(np.int64)
     root, rns = roots[0]
-    assert root == 0 and 1 <= rns <= 8
+    assert root == 0 and 1 <= rns <= width
     # structure: slots of a node tile its range contiguously; leaves tile [0, F)
     covered = np.zeros(F.shape[0], np.int64)
     stack = [(root, rns, 0, F.shape[0])]
@@ -62,7 +63,7 @@ def test_tree_structure_and_moments(oracle, leaf_max):
         node, ns, lo, cnt = stack.pop()
         pos = lo
         for s in range(ns):
-            k = node * 8 + s
+            k = node * width + s
             first, count, child, cns = it[k, 28], it[k, 29], it[k, 27], it[k, 30]
             assert first == pos and count >= 1
             pos += count
@@ -70,10 +71,10 @@ def test_tree_structure_and_moments(oracle, leaf_max):
                 assert count <= leaf_max
                 covered[first:first + count] += 1
             else:
-                assert 1 <= cns <= 8
+                assert 1 <= cns <= width
                 stack.append((child, cns, first, count))
-        for s in range(ns, 8):
-            assert it[node * 8 + s, 29] == 0
+        for s in range(ns, width):
+            assert it[node * width + s, 29] == 0
         assert pos == lo + cnt
     assert np.all(covered == 1)
     assert info["max_depth"] >= 1
@@ -114,15 +115,19 @@ def _slot_words(rec):
     return f
 
 
-def test_same_tree_parity_and_counters(oracle, open_mesh):
+@pytest.mark.parametrize("width,leaf_max", [(4, 1), (2, 1), (8, 1), (4, 4)])
+def test_same_tree_parity_and_counters(oracle, open_mesh, width, leaf_max):
     V, F = open_mesh
     lo, hi = V.min(0), V.max(0)
     q = np.concatenate([W.uniform_queries(lo, hi, 6000, 11, 1.3),
                         W.near_surface_queries(W.Mesh(V.astype(np.float64), F.astype(np.int64)), 2000, 12)])
-    e = FastWindingNumber((V, F))
+    e = FastWindingNumber((V, F), tree_width=width, leaf_max=leaf_max)
     om, st = e.query_stats(q)
+    om2 = e.solid_angle_batch(q)
+    np.testing.assert_array_equal(om, om2)  # the counting variant computes the same values
     slots, tris, roots, tau = e.export_tree()
-    ref, rst = oracle.tree_eval(slots, 8, int(roots[0, 0]), int(roots[0, 1]), tris, q, 2.0, float(tau[0]), stats=True)
+    ref, rst = oracle.tree_eval(slots, width, int(roots[0, 0]), int(roots[0, 1]), tris, q, 2.0, float(tau[0]),
+                                stats=True)
     np.testing.assert_array_equal(st, rst)
     assert np.abs(om.astype(np.float64) - ref).max() <= 2e-4
     assert st[:, 3].sum() > 0  # the near-surface points exercised the FP64 path
