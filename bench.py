@@ -77,15 +77,104 @@ def cpu_baseline(V, F, q, sample_stride: int, threads: int):
     }
 
 
+def bench_batched(steps: int = 5):
+    """64 meshes (displaced spheres / tori, 1k..500k triangles, 10k..2M points each, uniform in
+    bbox x1.3): one build and one query launch for all meshes, launch shape from the autotuner."""
+    import numpy as np
+    import torch
+
+    from lagrange_amd import workloads as W
+    from lagrange_amd.winding import MultiFastWindingNumber
+
+    meshes, pts = W.batched_workload()
+    t0 = time.perf_counter()
+    mm = MultiFastWindingNumber([(m.vertices, m.facets) for m in meshes])
+    build_wall = time.perf_counter() - t0
+    q_off = np.zeros(len(pts) + 1, np.int64)
+    q_off[1:] = np.cumsum([p.shape[0] for p in pts])
+    qd = torch.from_numpy(np.concatenate(pts)).cuda()
+    waves, mode, _ = mm.autotune(qd, q_off)
+    mm.solid_angle_batch(points=qd, q_off=q_off)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        om = mm.solid_angle_batch(points=qd, q_off=q_off)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    n = int(q_off[-1])
+    # accuracy on a strided sample of each of 8 meshes vs a per-mesh FP64 brute force engine
+    from lagrange_amd.winding import FastWindingNumber
+    errs = []
+    omh = om.cpu().numpy()
+    for i in range(0, len(meshes), 8):
+        sub = pts[i][:: max(1, pts[i].shape[0] // 512)]
+        wb = FastWindingNumber((meshes[i].vertices, meshes[i].facets)).bruteforce_f64(sub) / (4 * np.pi)
+        w = omh[q_off[i]:q_off[i + 1]][:: max(1, pts[i].shape[0] // 512)].astype(np.float64) / (4 * np.pi)
+        errs.append(np.abs(w - wb))
+    err = np.concatenate(errs)
+    info = mm.tree_info()
+    return {
+        "workload": f"{len(meshes)} meshes, {info['num_triangles']} triangles, {n} points (uniform in each bbox x1.3)",
+        "queries_per_s": n / dt,
+        "ms_per_step": dt * 1e3,
+        "launch": {"waves_per_block": waves, "mode": "persistent" if mode == 1 else "grid", "chosen_by": "autotune"},
+        "build_ms": info["build_ms"],
+        "build_wall_s": build_wall,
+        "max_abs_dw": float(err.max()),
+        "mean_abs_dw": float(err.mean()),
+    }
+
+
+def bench_near_surface(steps: int = 5):
+    """200,000-triangle displaced sphere, 4M points offset 1e-7..1e-2 x diag along normals."""
+    import numpy as np
+    import torch
+
+    from lagrange_amd import workloads as W
+    from lagrange_amd.winding import FastWindingNumber
+
+    m, q = W.near_surface_workload()
+    V, F = m.f32()
+    e = FastWindingNumber((V, F))
+    qd = torch.from_numpy(q).cuda()
+    e.solid_angle_batch(qd)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        om, ins = e.solid_angle_batch(qd, with_inside=True)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    idx = np.arange(0, q.shape[0], 977)
+    wb = e.bruteforce_f64(q[idx]) / (4 * np.pi)
+    w = om.cpu().numpy()[idx].astype(np.float64) / (4 * np.pi)
+    err = np.abs(w - wb)
+    insb = ins.cpu().numpy()[idx]
+    amb = np.abs(wb - 0.5) <= 1e-3
+    _, st = e.query_stats(q[idx])
+    return {
+        "workload": f"{F.shape[0]} triangles, {q.shape[0]} points at 1e-7..1e-2 x diag from the surface",
+        "queries_per_s": q.shape[0] / dt,
+        "ms_per_step": dt * 1e3,
+        "sample": int(idx.size),
+        "max_abs_dw": float(err.max()),
+        "p99_abs_dw": float(np.quantile(err, 0.99)),
+        "mean_abs_dw": float(err.mean()),
+        "inside_mismatch_excl_band": int(((insb != (wb > 0.5)) & ~amb).sum()),
+        "fp64_triangles_per_point": float(st[:, 3].mean()),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", type=int, default=256)
-    ap.add_argument("--leaf-max", type=int, default=1)
+    ap.add_argument("--leaf-max", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=4)
+    ap.add_argument("--extras", default="batched,near",
+                    help="comma list of extra workloads reported beside the headline: batched, near (or none)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (scale for rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -198,6 +287,11 @@ def main():
             "inside_mismatch_excl_band": int((~ins_ok & ~amb).sum()),
         },
     }
+    extras = [x for x in args.extras.split(",") if x and x != "none"]
+    if "batched" in extras:
+        line["batched"] = bench_batched()
+    if "near" in extras:
+        line["near_surface"] = bench_near_surface()
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
         threads = min(threads, 16)

@@ -58,14 +58,19 @@ typedef struct fwn_options {
                                  triangles use the FP64 solid angle when q is within tau of the
                                  leaf's bounding sphere and, for a single triangle, within tau of
                                  its plane */
-    int32_t leaf_max;         /* max triangles per leaf bucket (1..8, default 1: every triangle
-                                 carries its own far-field expansion, as the reference's items do) */
+    int32_t leaf_max;         /* max triangles per leaf bucket (1..8, default 4; 1 gives every
+                                 triangle its own far-field expansion, as the reference's items) */
     int32_t fp64_moment_min;  /* tree nodes with at least this many triangles merge moments in FP64 */
     int32_t device;           /* HIP device ordinal */
-    int32_t waves_per_block;  /* query launch shape (0 = default) */
+    int32_t waves_per_block;  /* query launch shape: 1, 2, 4 or 8 waves per block (default 1) */
     int32_t tree_width;       /* slots per tree node: 2, 4 or 8 (default 4) */
-    int32_t reserved[5];
+    int32_t launch_mode;      /* FWN_LAUNCH_GRID (default) or FWN_LAUNCH_PERSISTENT */
+    int32_t reserved[4];
 } fwn_options;
+
+/* query launch modes: one workgroup per group of 64-point batches, or a persistent grid whose
+ * waves take batches from a work counter, meshes with the deepest trees first */
+enum { FWN_LAUNCH_GRID = 0, FWN_LAUNCH_PERSISTENT = 1 };
 
 /* ---- exported tree format (fwn_export_tree) ------------------------------------------------
  * The engine stores, per mesh, a tree of nodes with tree_width (<= FWN_WIDTH) slots each; node
@@ -160,6 +165,18 @@ int fwn_query_stats(fwn_engine* e, const float* Q, const int64_t* q_off, int64_t
 /* FP64 brute force on the GPU: omega_out[i] = sum over all facets of mesh 0 of the exact
  * triangle solid angle (van Oosterom-Strackee, FP64).  Host pointers unless flags says device. */
 int fwn_bruteforce_f64(fwn_engine* e, const float* Q, int64_t nQ, double* omega_out, int32_t flags);
+
+/* Launch shape for later query calls on this engine (no reference counterpart: tuning hook).
+ * waves_per_block 1, 2, 4 or 8; mode FWN_LAUNCH_GRID or FWN_LAUNCH_PERSISTENT. */
+int fwn_set_launch(fwn_engine* e, int32_t waves_per_block, int32_t mode);
+
+/* Autotuner (no reference counterpart): times every launch shape (waves per block x grid or
+ * persistent) on the given points (same layout as fwn_solid_angle_multi; q_off NULL for a
+ * single-mesh engine), keeps the fastest on the engine, and returns it.  When cache_path is
+ * not NULL the choice is read from / appended to that text file, keyed by device name, tree
+ * width, leaf size, number of meshes and point count (log2 buckets). */
+int fwn_autotune(fwn_engine* e, const float* Q, const int64_t* q_off, int64_t nQ, int32_t flags,
+                 const char* cache_path, int32_t* waves_out, int32_t* mode_out, double* best_ms_out);
 
 /* Average device duration (ms) of the traversal kernel in the most recent query call and the
  * number of launches it took; measured with HIP events on the engine's stream. */

@@ -22,6 +22,8 @@ FWN_NO_DEVICE = 3
 FWN_HIP_ERROR = 4
 FWN_LIMIT_EXCEEDED = 5
 FWN_DEVICE_POINTERS = 1
+FWN_LAUNCH_GRID = 0
+FWN_LAUNCH_PERSISTENT = 1
 
 MAX_WIDTH = 8
 SLOT_WORDS = 32
@@ -40,7 +42,8 @@ class Options(C.Structure):
         ("device", C.c_int32),
         ("waves_per_block", C.c_int32),
         ("tree_width", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("launch_mode", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -103,6 +106,8 @@ def load():
         "fwn_query_stats": ([VP, VP, VP, I64, VP, VP, I32], I32),
         "fwn_bruteforce_f64": ([VP, VP, I64, VP, I32], I32),
         "fwn_last_timing": ([VP, VP, VP], I32),
+        "fwn_set_launch": ([VP, I32, I32], I32),
+        "fwn_autotune": ([VP, VP, VP, I64, I32, C.c_char_p, VP, VP, VP], I32),
         "fwn_last_error": ([], C.c_char_p),
     }
     for name, (args, res) in sig.items():

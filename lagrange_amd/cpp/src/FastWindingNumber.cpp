@@ -48,11 +48,13 @@ struct FastWindingNumber::Impl
         if (st != FWN_OK) raise("fwn_create", st);
     }
 
-    void check() const
-    {
-        if (!engine) throw Error("FastWindingNumber is not initialised");
-    }
 };
+
+// The engine of an initialised object; a default-constructed or moved-from object has no Impl
+// (the reference leaves such calls undefined; here they throw).
+#define FWN_ENGINE_OF(impl)                                                                    \
+    ((impl) && (impl)->engine ? (impl)->engine                                                 \
+                              : throw Error("FastWindingNumber is not initialised"))
 
 template <typename Scalar, typename Index>
 FastWindingNumber::FastWindingNumber(const SurfaceMesh<Scalar, Index>& mesh) : m_impl(make_value_ptr<Impl>())
@@ -69,35 +71,32 @@ FastWindingNumber& FastWindingNumber::operator=(FastWindingNumber&& other) noexc
 
 bool FastWindingNumber::is_inside(const std::array<float, 3>& pos) const
 {
-    m_impl->check();
     uint8_t r = 0;
-    int st = fwn_is_inside_point(m_impl->engine, pos.data(), &r);
+    int st = fwn_is_inside_point(FWN_ENGINE_OF(m_impl), pos.data(), &r);
     if (st != FWN_OK) raise("fwn_is_inside_point", st);
     return r != 0;
 }
 
 float FastWindingNumber::solid_angle(const std::array<float, 3>& pos) const
 {
-    m_impl->check();
     float r = 0.0f;
-    int st = fwn_solid_angle_point(m_impl->engine, pos.data(), &r);
+    int st = fwn_solid_angle_point(FWN_ENGINE_OF(m_impl), pos.data(), &r);
     if (st != FWN_OK) raise("fwn_solid_angle_point", st);
     return r;
 }
 
 void FastWindingNumber::solid_angle(const std::array<float, 3>* pos, size_t n, float* out) const
 {
-    m_impl->check();
-    int st = fwn_solid_angle(m_impl->engine, pos ? pos->data() : nullptr, (int64_t)n, out, nullptr, 0);
+    int st = fwn_solid_angle(FWN_ENGINE_OF(m_impl), pos ? pos->data() : nullptr, (int64_t)n, out, nullptr, 0);
     if (st != FWN_OK) raise("fwn_solid_angle", st);
 }
 
 void FastWindingNumber::is_inside(const std::array<float, 3>* pos, size_t n, bool* out) const
 {
-    m_impl->check();
+    fwn_engine* eng = FWN_ENGINE_OF(m_impl);
     std::vector<float> om(n);
     std::vector<uint8_t> in(n);
-    int st = fwn_solid_angle(m_impl->engine, pos ? pos->data() : nullptr, (int64_t)n, om.data(), in.data(), 0);
+    int st = fwn_solid_angle(eng, pos ? pos->data() : nullptr, (int64_t)n, om.data(), in.data(), 0);
     if (st != FWN_OK) raise("fwn_solid_angle", st);
     for (size_t i = 0; i < n; ++i) out[i] = in[i] != 0;
 }

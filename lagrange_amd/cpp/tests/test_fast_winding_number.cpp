@@ -169,7 +169,9 @@ int main()
              CHECK(a == b);
          }},
     };
+    std::setvbuf(stdout, nullptr, _IONBF, 0);
     for (auto& c : cases) {
+        std::printf("RUN  %s\n", c.first.c_str());
         int before = g_fail;
         try {
             c.second();

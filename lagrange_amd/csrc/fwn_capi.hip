@@ -5,8 +5,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <new>
+#include <sstream>
 #include <string>
 
 #include "fwn_internal.h"
@@ -55,6 +58,11 @@ void check_options(const fwn_options& o)
         fail(FWN_INVALID_ARGUMENT, "near_fp64_rel must be >= 0");
     if (o.leaf_max < 1 || o.leaf_max > 8) fail(FWN_INVALID_ARGUMENT, "leaf_max must be in [1, 8]");
     if (o.fp64_moment_min < 1) fail(FWN_INVALID_ARGUMENT, "fp64_moment_min must be >= 1");
+    if (o.waves_per_block != 0 && o.waves_per_block != 1 && o.waves_per_block != 2 && o.waves_per_block != 4 &&
+        o.waves_per_block != 8)
+        fail(FWN_INVALID_ARGUMENT, "waves_per_block must be 0 (default), 1, 2, 4 or 8");
+    if (o.launch_mode != FWN_LAUNCH_GRID && o.launch_mode != FWN_LAUNCH_PERSISTENT)
+        fail(FWN_INVALID_ARGUMENT, "launch_mode must be FWN_LAUNCH_GRID or FWN_LAUNCH_PERSISTENT");
     if (o.tree_width != 2 && o.tree_width != 4 && o.tree_width != 8)
         fail(FWN_INVALID_ARGUMENT, "tree_width must be 2, 4 or 8");
 }
@@ -180,11 +188,12 @@ int fwn_options_default(fwn_options* opt)
     std::memset(opt, 0, sizeof(*opt));
     opt->accuracy_scale = 2.0f;
     opt->near_fp64_rel = 1e-4f;
-    opt->leaf_max = 1;
+    opt->leaf_max = 4;
     opt->fp64_moment_min = 256;
     opt->device = 0;
-    opt->waves_per_block = 0;
+    opt->waves_per_block = 1;
     opt->tree_width = 4;
+    opt->launch_mode = FWN_LAUNCH_GRID;
     return FWN_OK;
 }
 
@@ -310,6 +319,95 @@ int fwn_bruteforce_f64(fwn_engine* h, const float* Q, int64_t nQ, double* omega_
         std::lock_guard<std::mutex> lk(e->mu);
         fwn::DeviceGuard g(e->device);
         fwn::run_bruteforce(*e, Q, nQ, omega_out, (flags & FWN_DEVICE_POINTERS) != 0);
+    });
+}
+
+int fwn_set_launch(fwn_engine* h, int32_t waves_per_block, int32_t mode)
+{
+    return guarded("fwn_set_launch", [&] {
+        fwn::Engine* e = E(h);
+        fwn_options o = e->opt;
+        o.waves_per_block = waves_per_block;
+        o.launch_mode = mode;
+        fwn::check_options(o);
+        std::lock_guard<std::mutex> lk(e->mu);
+        e->opt = o;
+    });
+}
+
+int fwn_autotune(fwn_engine* h, const float* Q, const int64_t* q_off, int64_t nQ, int32_t flags,
+                 const char* cache_path, int32_t* waves_out, int32_t* mode_out, double* best_ms_out)
+{
+    return guarded("fwn_autotune", [&] {
+        fwn::Engine* e = E(h);
+        std::lock_guard<std::mutex> lk(e->mu);
+        fwn::DeviceGuard g(e->device);
+        const bool dev = (flags & FWN_DEVICE_POINTERS) != 0;
+        if (q_off) nQ = q_off[e->n_meshes];
+        if (nQ <= 0 || !Q) fwn::fail(FWN_INVALID_ARGUMENT, "autotune needs points");
+        hipDeviceProp_t prop;
+        FWN_HIP(hipGetDeviceProperties(&prop, e->device));
+        int nq_bucket = 0;
+        while ((int64_t(1) << (nq_bucket + 1)) <= nQ) ++nq_bucket;
+        std::ostringstream ks;
+        ks << prop.gcnArchName << "|w" << e->width << "|l" << e->opt.leaf_max << "|m" << e->n_meshes << "|q" << nq_bucket;
+        const std::string key = ks.str();
+        int best_w = -1, best_m = -1;
+        double best = 1e300;
+        if (cache_path) {
+            std::ifstream in(cache_path);
+            std::string line;
+            while (std::getline(in, line)) {
+                std::istringstream ls(line);
+                std::string k;
+                int w, m;
+                double ms;
+                if (ls >> k >> w >> m >> ms && k == key) {
+                    best_w = w;
+                    best_m = m;
+                    best = ms;
+                }
+            }
+        }
+        if (best_w < 0) {
+            fwn::DBuf out;
+            std::vector<float> hout;
+            float* po = nullptr;
+            if (dev) {
+                out.alloc(sizeof(float) * nQ);
+                po = out.as<float>();
+            } else {
+                hout.resize(nQ);
+                po = hout.data();
+            }
+            const fwn_options saved = e->opt;
+            const int waves[4] = {1, 2, 4, 8};
+            for (int mode = FWN_LAUNCH_GRID; mode <= FWN_LAUNCH_PERSISTENT; ++mode)
+                for (int w : waves) {
+                    e->opt.waves_per_block = w;
+                    e->opt.launch_mode = mode;
+                    double t = 1e300;
+                    for (int rep = 0; rep < 3; ++rep) { // first run warms caches; keep the best of the rest
+                        fwn::run_queries(*e, Q, q_off, nQ, po, nullptr, nullptr, dev);
+                        if (rep > 0) t = std::min(t, e->last_total_ms);
+                    }
+                    if (t < best) {
+                        best = t;
+                        best_w = w;
+                        best_m = mode;
+                    }
+                }
+            e->opt = saved;
+            if (cache_path) {
+                std::ofstream o(cache_path, std::ios::app);
+                o << key << " " << best_w << " " << best_m << " " << best << "\n";
+            }
+        }
+        e->opt.waves_per_block = best_w;
+        e->opt.launch_mode = best_m;
+        if (waves_out) *waves_out = best_w;
+        if (mode_out) *mode_out = best_m;
+        if (best_ms_out) *best_ms_out = best;
     });
 }
 

@@ -80,7 +80,8 @@ struct Engine {
 
     // query scratch (grown on demand)
     DBuf q_keys[2], q_vals[2], q_hist, q_rowsum, q_in, q_om, q_in8, q_off, q_boff, q_stats, q_flag;
-    DBuf q_bf;
+    DBuf q_bf, q_order;
+    int n_cu = 0;
     // deferred FP64 near-field work: (sorted point, first triangle, count) items and per-point
     // fixed-point accumulators (zero between calls)
     DBuf q_items, q_acc;
@@ -97,6 +98,9 @@ struct Engine {
 // (swapping with the scratch buffers as needed; the pointers are updated).
 void radix_sort_pairs(uint64_t*& keys, uint32_t*& vals, uint64_t*& keys_alt, uint32_t*& vals_alt, int64_t n,
                       int nbits, DBuf& hist, DBuf& rowsum, hipStream_t s);
+// Same for 32-bit keys, sorting key bits [begin_bit, end_bit) only (low bits keep input order).
+void radix_sort_pairs32(uint32_t*& keys, uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt, int64_t n,
+                        int begin_bit, int end_bit, DBuf& hist, DBuf& rowsum, hipStream_t s);
 // exclusive scan of int32 (in may alias out); returns nothing, total in out[n] if out has n+1 slots
 void exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, DBuf& tmp, hipStream_t s);
 

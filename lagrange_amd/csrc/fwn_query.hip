@@ -41,9 +41,13 @@ __device__ __forceinline__ int mesh_of64(const int64_t* off, int M, int64_t t)
     return lo;
 }
 
+// 32-bit point keys: mesh id in the top mesh_bits bits, then the leading bits of the point's
+// 30-bit Morton code in its mesh's box.  Only the 24 top bits are sorted: a wave needs its 64
+// points close together, not in exact curve order, and every point's result is independent of
+// which points share its wave.
 __global__ __launch_bounds__(QT) void k_query_keys(const float* __restrict__ Q, int64_t nQ,
-                                                   const int64_t* __restrict__ q_off, int M,
-                                                   const float* __restrict__ minfo, uint64_t* __restrict__ keys,
+                                                   const int64_t* __restrict__ q_off, int M, int mesh_bits,
+                                                   const float* __restrict__ minfo, uint32_t* __restrict__ keys,
                                                    uint32_t* __restrict__ vals)
 {
     int64_t i = (int64_t)blockIdx.x * QT + threadIdx.x;
@@ -51,7 +55,10 @@ __global__ __launch_bounds__(QT) void k_query_keys(const float* __restrict__ Q, 
     int m = (M > 1) ? mesh_of64(q_off, M, i) : 0;
     const float* mi = minfo + 8 * m;
     uint32_t mc = morton30(Q[3 * i], Q[3 * i + 1], Q[3 * i + 2], mi, mi + 3);
-    keys[i] = ((uint64_t)(uint32_t)m << 32) | mc;
+    const int mbits = 32 - mesh_bits; // bits left for the Morton code
+    uint32_t key = (mbits >= 30) ? mc : (mc >> (30 - mbits));
+    if (mesh_bits > 0) key |= (uint32_t)m << mbits;
+    keys[i] = key;
     vals[i] = (uint32_t)i;
 }
 
@@ -122,6 +129,43 @@ __device__ __forceinline__ float far_field(const float* __restrict__ w, float dx
     return fmaf(o0 + w[FWN_W_TR], inv3, fmaf(q1, inv5, q2 * inv7));
 }
 
+// One slot's 32 words held in SGPRs (two 16-dword scalar loads).  The traversal issues the
+// next slot's loads before the current slot's arithmetic so the load latency overlaps it.
+typedef uint32_t v16u __attribute__((ext_vector_type(16)));
+struct SlotRegs {
+    v16u a, b;
+};
+
+__device__ __forceinline__ SlotRegs load_slot(const uint32_t* __restrict__ p)
+{
+    const v16u* q = reinterpret_cast<const v16u*>(p);
+    return SlotRegs{q[0], q[1]};
+}
+
+__device__ __forceinline__ float sw(const SlotRegs& r, int k) { return __uint_as_float(k < 16 ? r.a[k] : r.b[k - 16]); }
+__device__ __forceinline__ int32_t si(const SlotRegs& r, int k) { return (int32_t)(k < 16 ? r.a[k] : r.b[k - 16]); }
+
+// order-2 far field of one slot held in registers
+__device__ __forceinline__ float far_field_r(const SlotRegs& r, float dx, float dy, float dz, float d2)
+{
+    float inv = __builtin_amdgcn_rsqf(d2);
+    float inv2 = inv * inv;
+    float inv3 = inv2 * inv;
+    float inv5 = inv3 * inv2;
+    float inv7 = inv5 * inv2;
+    float o0 = fmaf(dx, sw(r, FWN_W_NX), fmaf(dy, sw(r, FWN_W_NY), dz * sw(r, FWN_W_NZ)));
+    float xx = dx * dx, yy = dy * dy, zz = dz * dz;
+    float xy = dx * dy, yz = dy * dz, zx = dz * dx;
+    float q1 = fmaf(sw(r, FWN_W_A), xx, fmaf(sw(r, FWN_W_A + 1), yy, fmaf(sw(r, FWN_W_A + 2), zz,
+               fmaf(sw(r, FWN_W_A + 3), xy, fmaf(sw(r, FWN_W_A + 4), yz, sw(r, FWN_W_A + 5) * zx)))));
+    q1 = fmaf(sw(r, FWN_W_C), dx, fmaf(sw(r, FWN_W_C + 1), dy, fmaf(sw(r, FWN_W_C + 2), dz, q1)));
+    float q2 = fmaf(sw(r, FWN_W_B), xx * dx, fmaf(sw(r, FWN_W_B + 1), yy * dy, fmaf(sw(r, FWN_W_B + 2), zz * dz,
+               fmaf(sw(r, FWN_W_B + 3), xx * dy, fmaf(sw(r, FWN_W_B + 4), xx * dz, fmaf(sw(r, FWN_W_B + 5), yy * dx,
+               fmaf(sw(r, FWN_W_B + 6), yy * dz, fmaf(sw(r, FWN_W_B + 7), zz * dx,
+               fmaf(sw(r, FWN_W_B + 8), zz * dy, sw(r, FWN_W_B + 9) * (xy * dz))))))))));
+    return fmaf(o0 + sw(r, FWN_W_TR), inv3, fmaf(q1, inv5, q2 * inv7));
+}
+
 // ---- wave stack in VGPRs: entry i sits in lane (i & 63) of bank (i >> 6) -----------------
 // Named registers (no arrays) so the banks stay in VGPRs rather than private memory.
 struct WaveStack {
@@ -161,21 +205,21 @@ __device__ __forceinline__ void stack_pop(const WaveStack& S, int sp, uint32_t& 
     mk = ((uint64_t)hi << 32) | lo;
 }
 
-template <bool STATS, int WAVES>
-__global__ __launch_bounds__(WAVES * 64) void k_traverse(
-    const uint32_t* __restrict__ slots, int width, const uint32_t* __restrict__ tris,
+// One batch of up to 64 points (one per lane) of one mesh, walked through that mesh's tree.
+template <bool STATS>
+__device__ __forceinline__ void traverse_batch(
+    int64_t batch, const uint32_t* __restrict__ slots, int width, const uint32_t* __restrict__ tris,
     const int32_t* __restrict__ roots, const float* __restrict__ minfo, const float* __restrict__ Q,
-    const uint32_t* __restrict__ perm, const int64_t* __restrict__ q_off, const int64_t* __restrict__ b_off, int M,
-    int64_t n_batches, float beta2, float* __restrict__ omega, uint8_t* __restrict__ inside,
-    int32_t* __restrict__ stats, uint32_t* __restrict__ items, uint32_t* __restrict__ n_items, uint32_t item_cap,
-    int32_t* __restrict__ flag)
+    const uint32_t* __restrict__ perm, const int64_t* __restrict__ q_off, const int64_t* __restrict__ b_off,
+    const int32_t* __restrict__ mesh_order, int M, float beta2, float* __restrict__ omega,
+    uint8_t* __restrict__ inside, int32_t* __restrict__ stats, uint32_t* __restrict__ items,
+    uint32_t* __restrict__ n_items, uint32_t item_cap, int32_t* __restrict__ flag)
 {
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int64_t batch = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * WAVES + wave;
-    if (batch >= n_batches) return; // whole wave
-    const int m = mesh_of64(b_off, M, batch);
-    const int64_t start = q_off[m] + (batch - b_off[m]) * 64;
+    // batches are numbered over the meshes in mesh_order; b_off is the prefix over that order
+    const int pos = mesh_of64(b_off, M, batch);
+    const int m = mesh_order[pos];
+    const int64_t start = q_off[m] + (batch - b_off[pos]) * 64;
     const int64_t cnt64 = q_off[m + 1] - start;
     const int cnt = cnt64 < 64 ? (int)cnt64 : 64;
     const bool valid = lane < cnt;
@@ -205,63 +249,68 @@ __global__ __launch_bounds__(WAVES * 64) void k_traverse(
             const bool act = (mk >> lane) & 1ull;
             const uint32_t* nb = slots + (size_t)(e >> 4) * (size_t)width * kSlotWords;
             const int ns = (int)(e & 15u);
+            SlotRegs cur = load_slot(nb);
             for (int s = 0; s < ns; ++s) {
-                const float* w = reinterpret_cast<const float*>(nb + s * kSlotWords);
-                const float dx = w[FWN_W_PX] - qx, dy = w[FWN_W_PY] - qy, dz = w[FWN_W_PZ] - qz;
+                SlotRegs nxt = cur;
+                if (s + 1 < ns) nxt = load_slot(nb + (s + 1) * kSlotWords);
+                const float dx = sw(cur, FWN_W_PX) - qx, dy = sw(cur, FWN_W_PY) - qy, dz = sw(cur, FWN_W_PZ) - qz;
                 const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                const float r2 = w[FWN_W_R2];
+                const float r2 = sw(cur, FWN_W_R2);
                 const bool far = d2 > beta2 * r2;
                 if (STATS) n_test += act ? 1 : 0;
                 if (act && far) {
-                    om += far_field(w, dx, dy, dz, d2);
+                    om += far_field_r(cur, dx, dy, dz, d2);
                     if (STATS) ++n_far;
                 }
                 const uint64_t nearm = __ballot(act && !far);
-                if (nearm == 0ull) continue;
-                const int32_t child = __float_as_int(w[FWN_W_CHILD]);
-                if (child < 0) {
-                    const int32_t first = __float_as_int(w[FWN_W_FIRST]);
-                    const int32_t tcnt = __float_as_int(w[FWN_W_COUNT]);
-                    const bool nl = (nearm >> lane) & 1ull;
-                    // FP64 for points within tau of the leaf's bounding sphere and, for a single
-                    // triangle, within tau of its plane (p lies on that plane)
-                    const float rr = sqrtf(r2) + tau;
-                    const float nd = fmaf(dx, w[FWN_W_NX], fmaf(dy, w[FWN_W_NY], dz * w[FWN_W_NZ]));
-                    const float nn = fmaf(w[FWN_W_NX], w[FWN_W_NX], fmaf(w[FWN_W_NY], w[FWN_W_NY], w[FWN_W_NZ] * w[FWN_W_NZ]));
-                    const bool use64 = nl && tau > 0.0f && d2 < rr * rr && (tcnt > 1 || nd * nd < tau2 * nn);
-                    const bool use32 = nl && !use64;
-                    if (__ballot(use32) != 0ull) {
-                        for (int32_t t = first; t < first + tcnt; ++t) {
-                            const float* tw = reinterpret_cast<const float*>(tris + (size_t)t * kTriWords);
-                            if (use32) {
-                                om += tri_f32(tw, qx, qy, qz);
-                                if (STATS) ++n_t32;
+                if (nearm != 0ull) {
+                    const int32_t child = si(cur, FWN_W_CHILD);
+                    if (child < 0) {
+                        const int32_t first = si(cur, FWN_W_FIRST);
+                        const int32_t tcnt = si(cur, FWN_W_COUNT);
+                        const bool nl = (nearm >> lane) & 1ull;
+                        // FP64 for points within tau of the leaf's bounding sphere and, for a single
+                        // triangle, within tau of its plane (p lies on that plane)
+                        const float rr = sqrtf(r2) + tau;
+                        const float nx = sw(cur, FWN_W_NX), ny = sw(cur, FWN_W_NY), nz = sw(cur, FWN_W_NZ);
+                        const float nd = fmaf(dx, nx, fmaf(dy, ny, dz * nz));
+                        const float nn = fmaf(nx, nx, fmaf(ny, ny, nz * nz));
+                        const bool use64 = nl && tau > 0.0f && d2 < rr * rr && (tcnt > 1 || nd * nd < tau2 * nn);
+                        const bool use32 = nl && !use64;
+                        if (__ballot(use32) != 0ull) {
+                            for (int32_t t = first; t < first + tcnt; ++t) {
+                                const float* tw = reinterpret_cast<const float*>(tris + (size_t)t * kTriWords);
+                                if (use32) {
+                                    om += tri_f32(tw, qx, qy, qz);
+                                    if (STATS) ++n_t32;
+                                }
                             }
                         }
-                    }
-                    const uint64_t m64 = __ballot(use64);
-                    if (m64 != 0ull) {
-                        // deferred: (point, leaf) items, evaluated in FP64 by k_near64
-                        uint32_t base = 0u;
-                        if (lane == 0) base = atomicAdd(n_items, (uint32_t)__popcll(m64));
-                        base = __builtin_amdgcn_readfirstlane(base);
-                        if (use64) {
-                            const uint32_t k = base + (uint32_t)__popcll(m64 & lt_mask);
-                            if (k < item_cap) {
-                                items[3 * (size_t)k] = jpos;
-                                items[3 * (size_t)k + 1] = (uint32_t)first;
-                                items[3 * (size_t)k + 2] = (uint32_t)tcnt;
+                        const uint64_t m64 = __ballot(use64);
+                        if (m64 != 0ull) {
+                            // deferred: (point, leaf) items, evaluated in FP64 by k_near64
+                            uint32_t base = 0u;
+                            if (lane == 0) base = atomicAdd(n_items, (uint32_t)__popcll(m64));
+                            base = __builtin_amdgcn_readfirstlane(base);
+                            if (use64) {
+                                const uint32_t k = base + (uint32_t)__popcll(m64 & lt_mask);
+                                if (k < item_cap) {
+                                    items[3 * (size_t)k] = jpos;
+                                    items[3 * (size_t)k + 1] = (uint32_t)first;
+                                    items[3 * (size_t)k + 2] = (uint32_t)tcnt;
+                                }
+                                if (STATS) n_t64 += tcnt;
                             }
-                            if (STATS) n_t64 += tcnt;
                         }
+                    } else if (sp < kStack) {
+                        const uint32_t cns = (uint32_t)si(cur, FWN_W_CHILD_NS);
+                        stack_push(S, sp, lane, ((uint32_t)child << 4) | cns, nearm);
+                        ++sp;
+                    } else {
+                        overflow = true;
                     }
-                } else if (sp < kStack) {
-                    const uint32_t cns = (uint32_t)__float_as_int(w[FWN_W_CHILD_NS]);
-                    stack_push(S, sp, lane, ((uint32_t)child << 4) | cns, nearm);
-                    ++sp;
-                } else {
-                    overflow = true;
                 }
+                cur = nxt;
             }
         }
         if (overflow && lane == 0) atomicOr(flag, 1);
@@ -274,6 +323,35 @@ __global__ __launch_bounds__(WAVES * 64) void k_traverse(
             stats[4 * (int64_t)orig + 1] = n_far;
             stats[4 * (int64_t)orig + 2] = n_t32;
             stats[4 * (int64_t)orig + 3] = n_t64;
+        }
+    }
+}
+
+// Grid launch: one wave per batch, XCD-aware block order.  Persistent launch: a grid sized to
+// the device's resident waves; each wave takes batches from a work counter until none is left
+// (the batch numbering puts the meshes with the most triangles first).
+template <bool STATS, int WAVES, bool PERSIST>
+__global__ __launch_bounds__(WAVES * 64) void k_traverse(
+    const uint32_t* __restrict__ slots, int width, const uint32_t* __restrict__ tris,
+    const int32_t* __restrict__ roots, const float* __restrict__ minfo, const float* __restrict__ Q,
+    const uint32_t* __restrict__ perm, const int64_t* __restrict__ q_off, const int64_t* __restrict__ b_off,
+    const int32_t* __restrict__ mesh_order, int M, int64_t n_batches, float beta2, float* __restrict__ omega,
+    uint8_t* __restrict__ inside, int32_t* __restrict__ stats, uint32_t* __restrict__ items,
+    uint32_t* __restrict__ n_items, uint32_t item_cap, int32_t* __restrict__ flag, uint32_t* __restrict__ work)
+{
+    if (!PERSIST) {
+        const int64_t batch = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * WAVES + (threadIdx.x >> 6);
+        if (batch >= n_batches) return; // whole wave
+        traverse_batch<STATS>(batch, slots, width, tris, roots, minfo, Q, perm, q_off, b_off, mesh_order, M, beta2,
+                              omega, inside, stats, items, n_items, item_cap, flag);
+    } else {
+        for (;;) {
+            uint32_t b = 0u;
+            if ((threadIdx.x & 63) == 0) b = atomicAdd(work, 1u);
+            b = __builtin_amdgcn_readfirstlane(b);
+            if ((int64_t)b >= n_batches) break; // every wave leaves once the counter passes the end
+            traverse_batch<STATS>((int64_t)b, slots, width, tris, roots, minfo, Q, perm, q_off, b_off, mesh_order, M,
+                                  beta2, omega, inside, stats, items, n_items, item_cap, flag);
         }
     }
 }
@@ -338,10 +416,54 @@ __global__ __launch_bounds__(256) void k_bruteforce(const uint32_t* __restrict__
     if (valid) out[i] = s;
 }
 
-__global__ void k_zero2(int32_t* a, uint32_t* b)
+__global__ void k_zero3(int32_t* a)
 {
-    *a = 0;
-    *b = 0u;
+    a[0] = 0;
+    a[1] = 0;
+    a[2] = 0;
+}
+
+template <bool ST, int WV, bool PE>
+void* traverse_fn()
+{
+    return reinterpret_cast<void*>(&k_traverse<ST, WV, PE>);
+}
+
+template <typename... Args>
+void launch_traverse(bool stats, int waves, bool persist, unsigned nblk, hipStream_t s, Args... args)
+{
+#define FWN_TRAV(ST, WV, PE) hipLaunchKernelGGL((k_traverse<ST, WV, PE>), dim3(nblk), dim3(WV * 64), 0, s, args...)
+    if (stats) {
+        FWN_TRAV(true, 1, false);
+        return;
+    }
+    switch (waves * 2 + (persist ? 1 : 0)) {
+    case 2: FWN_TRAV(false, 1, false); break;
+    case 3: FWN_TRAV(false, 1, true); break;
+    case 4: FWN_TRAV(false, 2, false); break;
+    case 5: FWN_TRAV(false, 2, true); break;
+    case 9: FWN_TRAV(false, 4, true); break;
+    case 16: FWN_TRAV(false, 8, false); break;
+    case 17: FWN_TRAV(false, 8, true); break;
+    default: FWN_TRAV(false, 4, false); break;
+    }
+#undef FWN_TRAV
+}
+
+// resident workgroups per CU of a traversal variant (persistent grid size)
+int resident_blocks(bool persist, int waves)
+{
+    const void* f = nullptr;
+    switch (waves * 2 + (persist ? 1 : 0)) {
+    case 3: f = traverse_fn<false, 1, true>(); break;
+    case 5: f = traverse_fn<false, 2, true>(); break;
+    case 9: f = traverse_fn<false, 4, true>(); break;
+    case 17: f = traverse_fn<false, 8, true>(); break;
+    default: return 1;
+    }
+    int nb = 0;
+    FWN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, waves * 64, 0));
+    return nb > 0 ? nb : 1;
 }
 
 } // namespace
@@ -369,8 +491,15 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
     if (1 + (int64_t)(e.width - 1) * e.max_depth > kStack)
         fail(FWN_LIMIT_EXCEEDED, "tree deeper than the traversal stack supports");
 
+    // batches of 64 points, numbered over the meshes in decreasing triangle count so that a
+    // persistent launch starts the deepest trees first
+    std::vector<int32_t> order(M);
+    for (int m = 0; m < M; ++m) order[m] = m;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        return e.h_f_off[a + 1] - e.h_f_off[a] > e.h_f_off[b + 1] - e.h_f_off[b];
+    });
     std::vector<int64_t> boff(M + 1, 0);
-    for (int m = 0; m < M; ++m) boff[m + 1] = boff[m] + (qoff[m + 1] - qoff[m] + 63) / 64;
+    for (int k = 0; k < M; ++k) boff[k + 1] = boff[k] + (qoff[order[k] + 1] - qoff[order[k]] + 63) / 64;
     const int64_t nb = boff[M];
 
     const float* dQ = Q;
@@ -394,12 +523,14 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
     }
     e.q_off.reserve(sizeof(int64_t) * (M + 1));
     e.q_boff.reserve(sizeof(int64_t) * (M + 1));
-    e.q_flag.reserve(2 * sizeof(int32_t));
+    e.q_order.reserve(sizeof(int32_t) * M);
+    e.q_flag.reserve(3 * sizeof(int32_t));
     FWN_HIP(hipMemcpyAsync(e.q_off.p, qoff.data(), sizeof(int64_t) * (M + 1), hipMemcpyHostToDevice, s));
     FWN_HIP(hipMemcpyAsync(e.q_boff.p, boff.data(), sizeof(int64_t) * (M + 1), hipMemcpyHostToDevice, s));
+    FWN_HIP(hipMemcpyAsync(e.q_order.p, order.data(), sizeof(int32_t) * M, hipMemcpyHostToDevice, s));
 
     for (int k = 0; k < 2; ++k) {
-        e.q_keys[k].reserve(sizeof(uint64_t) * nQ);
+        e.q_keys[k].reserve(sizeof(uint32_t) * nQ);
         e.q_vals[k].reserve(sizeof(uint32_t) * nQ);
     }
     if (!e.ev[0])
@@ -418,37 +549,42 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
     }
 
     FWN_HIP(hipEventRecord(e.ev[0], s));
-    uint64_t* kp = e.q_keys[0].as<uint64_t>();
+    uint32_t* kp = e.q_keys[0].as<uint32_t>();
     uint32_t* vp = e.q_vals[0].as<uint32_t>();
-    uint64_t* kp2 = e.q_keys[1].as<uint64_t>();
+    uint32_t* kp2 = e.q_keys[1].as<uint32_t>();
     uint32_t* vp2 = e.q_vals[1].as<uint32_t>();
-    hipLaunchKernelGGL(k_query_keys, dim3(grid_blocks(nQ, QT)), dim3(QT), 0, s, dQ, nQ, e.q_off.as<int64_t>(), M,
-                       e.mesh_info.as<float>(), kp, vp);
-    FWN_LAUNCH_CHECK("k_query_keys");
     int mesh_bits = 0;
     while ((1 << mesh_bits) < M) ++mesh_bits;
-    radix_sort_pairs(kp, vp, kp2, vp2, nQ, (M > 1) ? 32 + mesh_bits : 30, e.q_hist, e.q_rowsum, s);
+    if (mesh_bits > 24) fail(FWN_LIMIT_EXCEEDED, "more than 2^24 meshes in one engine");
+    hipLaunchKernelGGL(k_query_keys, dim3(grid_blocks(nQ, QT)), dim3(QT), 0, s, dQ, nQ, e.q_off.as<int64_t>(), M,
+                       mesh_bits, e.mesh_info.as<float>(), kp, vp);
+    FWN_LAUNCH_CHECK("k_query_keys");
+    radix_sort_pairs32(kp, vp, kp2, vp2, nQ, 8, 32, e.q_hist, e.q_rowsum, s);
 
-    constexpr int WAVES = 4;
-    const int64_t nblk = (nb + WAVES - 1) / WAVES;
+    const int waves = e.opt.waves_per_block > 0 ? e.opt.waves_per_block : 1;
+    const bool persist = (e.opt.launch_mode == FWN_LAUNCH_PERSISTENT) && !stats;
+    int64_t nblk = (nb + waves - 1) / waves;
+    if (persist) {
+        if (e.n_cu == 0) {
+            hipDeviceProp_t prop;
+            FWN_HIP(hipGetDeviceProperties(&prop, e.device));
+            e.n_cu = prop.multiProcessorCount;
+        }
+        nblk = std::min<int64_t>(nblk, (int64_t)e.n_cu * resident_blocks(true, waves));
+    }
     if (nblk > INT32_MAX) fail(FWN_LIMIT_EXCEEDED, "too many point batches");
     const float beta2 = e.opt.accuracy_scale * e.opt.accuracy_scale;
     int32_t* dflag = e.q_flag.as<int32_t>();
     uint32_t* dnitems = reinterpret_cast<uint32_t*>(dflag + 1);
+    uint32_t* dwork = reinterpret_cast<uint32_t*>(dflag + 2);
     for (int attempt = 0;; ++attempt) {
-        hipLaunchKernelGGL(k_zero2, dim3(1), dim3(1), 0, s, dflag, dnitems);
+        hipLaunchKernelGGL(k_zero3, dim3(1), dim3(1), 0, s, dflag);
         FWN_HIP(hipEventRecord(e.ev[1], s));
         const uint32_t cap = (uint32_t)e.item_cap;
-        if (stats)
-            hipLaunchKernelGGL((k_traverse<true, WAVES>), dim3((unsigned)nblk), dim3(WAVES * 64), 0, s,
-                               e.slots.as<uint32_t>(), e.width, e.tris.as<uint32_t>(), e.roots.as<int32_t>(),
-                               e.mesh_info.as<float>(), dQ, vp, e.q_off.as<int64_t>(), e.q_boff.as<int64_t>(), M, nb,
-                               beta2, dOm, dIn, dSt, e.q_items.as<uint32_t>(), dnitems, cap, dflag);
-        else
-            hipLaunchKernelGGL((k_traverse<false, WAVES>), dim3((unsigned)nblk), dim3(WAVES * 64), 0, s,
-                               e.slots.as<uint32_t>(), e.width, e.tris.as<uint32_t>(), e.roots.as<int32_t>(),
-                               e.mesh_info.as<float>(), dQ, vp, e.q_off.as<int64_t>(), e.q_boff.as<int64_t>(), M, nb,
-                               beta2, dOm, dIn, dSt, e.q_items.as<uint32_t>(), dnitems, cap, dflag);
+        launch_traverse(stats != nullptr, waves, persist, (unsigned)nblk, s, e.slots.as<uint32_t>(), e.width,
+                        e.tris.as<uint32_t>(), e.roots.as<int32_t>(), e.mesh_info.as<float>(), dQ, vp,
+                        e.q_off.as<int64_t>(), e.q_boff.as<int64_t>(), e.q_order.as<int32_t>(), M, nb, beta2, dOm,
+                        dIn, dSt, e.q_items.as<uint32_t>(), dnitems, cap, dflag, dwork);
         FWN_LAUNCH_CHECK("k_traverse");
         FWN_HIP(hipEventRecord(e.ev[2], s));
         hipLaunchKernelGGL(k_near64, dim3(2048), dim3(256), 0, s, e.q_items.as<uint32_t>(), dnitems, cap,

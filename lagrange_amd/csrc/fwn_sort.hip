@@ -1,4 +1,4 @@
-// Stable LSD radix sort of (uint64 key, uint32 value) pairs and an int32 exclusive scan,
+// Stable LSD radix sort of (uint64 or uint32 key, uint32 value) pairs and an int32 exclusive scan,
 // written for gfx950 wave64.  Used for the Morton order of triangles (tree build) and of
 // query points (wave batches of nearby points).
 //
@@ -20,7 +20,8 @@ constexpr int RS_ROUNDS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;
 constexpr int RS_WAVES = RS_THREADS / kWave;
 
-__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* __restrict__ keys, int64_t n, int shift,
+template <typename KeyT>
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const KeyT* __restrict__ keys, int64_t n, int shift,
                                                         int32_t* __restrict__ hist, int nblocks)
 {
     __shared__ int32_t cnt[256];
@@ -75,9 +76,10 @@ __global__ __launch_bounds__(256) void k_rs_digits(int32_t* __restrict__ rowsum)
     rowsum[threadIdx.x] = part[threadIdx.x] - v;
 }
 
-__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __restrict__ kin,
+template <typename KeyT>
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const KeyT* __restrict__ kin,
                                                            const uint32_t* __restrict__ vin,
-                                                           uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                           KeyT* __restrict__ kout, uint32_t* __restrict__ vout,
                                                            int64_t n, int shift, const int32_t* __restrict__ hist,
                                                            const int32_t* __restrict__ digit_base, int nblocks)
 {
@@ -94,7 +96,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __res
         for (int k = tid; k < RS_WAVES * 256; k += RS_THREADS) (&wc[0][0])[k] = 0;
         int64_t i = base + r * RS_THREADS + tid;
         bool valid = i < n;
-        uint64_t key = valid ? kin[i] : 0ull;
+        KeyT key = valid ? kin[i] : (KeyT)0;
         uint32_t val = valid ? vin[i] : 0u;
         uint32_t d = (uint32_t)(key >> shift) & 255u;
         uint64_t peers = __ballot(valid);
@@ -186,31 +188,44 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_add(int32_t* __restrict__ o
     if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = tile_sums[gridDim.x];
 }
 
-} // namespace
-
-void radix_sort_pairs(uint64_t*& keys, uint32_t*& vals, uint64_t*& keys_alt, uint32_t*& vals_alt, int64_t n,
-                      int nbits, DBuf& hist, DBuf& rowsum, hipStream_t s)
+template <typename KeyT>
+void radix_sort_impl(KeyT*& keys, uint32_t*& vals, KeyT*& keys_alt, uint32_t*& vals_alt, int64_t n, int begin_bit,
+                     int end_bit, DBuf& hist, DBuf& rowsum, hipStream_t s)
 {
     if (n <= 1) return;
     if (n > (int64_t)INT32_MAX) fail(FWN_LIMIT_EXCEEDED, "radix sort: more than 2^31-1 elements");
     int nblocks = (int)((n + RS_TILE - 1) / RS_TILE);
     hist.reserve(sizeof(int32_t) * 256 * (size_t)nblocks);
     rowsum.reserve(sizeof(int32_t) * 256);
-    for (int shift = 0; shift < nbits; shift += 8) {
-        hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(RS_THREADS), 0, s, keys, n, shift, hist.as<int32_t>(),
-                           nblocks);
+    for (int shift = begin_bit; shift < end_bit; shift += 8) {
+        hipLaunchKernelGGL(k_rs_hist<KeyT>, dim3(nblocks), dim3(RS_THREADS), 0, s, keys, n, shift,
+                           hist.as<int32_t>(), nblocks);
         FWN_LAUNCH_CHECK("k_rs_hist");
         hipLaunchKernelGGL(k_rs_rows, dim3(256), dim3(256), 0, s, hist.as<int32_t>(), nblocks,
                            rowsum.as<int32_t>());
         FWN_LAUNCH_CHECK("k_rs_rows");
         hipLaunchKernelGGL(k_rs_digits, dim3(1), dim3(256), 0, s, rowsum.as<int32_t>());
         FWN_LAUNCH_CHECK("k_rs_digits");
-        hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(RS_THREADS), 0, s, keys, vals, keys_alt, vals_alt, n,
-                           shift, hist.as<int32_t>(), rowsum.as<int32_t>(), nblocks);
+        hipLaunchKernelGGL(k_rs_scatter<KeyT>, dim3(nblocks), dim3(RS_THREADS), 0, s, keys, vals, keys_alt,
+                           vals_alt, n, shift, hist.as<int32_t>(), rowsum.as<int32_t>(), nblocks);
         FWN_LAUNCH_CHECK("k_rs_scatter");
         std::swap(keys, keys_alt);
         std::swap(vals, vals_alt);
     }
+}
+
+} // namespace
+
+void radix_sort_pairs(uint64_t*& keys, uint32_t*& vals, uint64_t*& keys_alt, uint32_t*& vals_alt, int64_t n,
+                      int nbits, DBuf& hist, DBuf& rowsum, hipStream_t s)
+{
+    radix_sort_impl<uint64_t>(keys, vals, keys_alt, vals_alt, n, 0, nbits, hist, rowsum, s);
+}
+
+void radix_sort_pairs32(uint32_t*& keys, uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt, int64_t n,
+                        int begin_bit, int end_bit, DBuf& hist, DBuf& rowsum, hipStream_t s)
+{
+    radix_sort_impl<uint32_t>(keys, vals, keys_alt, vals_alt, n, begin_bit, end_bit, hist, rowsum, s);
 }
 
 // out must hold n+1 entries; out[n] receives the total.

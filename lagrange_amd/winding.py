@@ -89,6 +89,26 @@ class _EngineBase:
                                          tau.ctypes.data))
         return slots, tris, roots, tau
 
+    def set_launch(self, waves_per_block: int, mode: int = L.FWN_LAUNCH_GRID):
+        L.check(L.load().fwn_set_launch(self._h(), waves_per_block, mode))
+
+    def autotune(self, points, q_off=None, cache_path: str | None = None):
+        """Time every launch shape on these points and keep the fastest: (waves, mode, ms)."""
+        w, m, ms = C.c_int32(), C.c_int32(), C.c_double()
+        qo = None if q_off is None else np.ascontiguousarray(q_off, dtype=np.int64)
+        if _is_torch(points) and points.is_cuda:
+            import torch
+            q = points.contiguous().to(torch.float32)
+            torch.cuda.synchronize(q.device)
+            n, ptr, fl = q.shape[0], _ptr(q), L.FWN_DEVICE_POINTERS
+        else:
+            q = np.ascontiguousarray(np.asarray(points, dtype=np.float32).reshape(-1, 3))
+            n, ptr, fl = q.shape[0], q.ctypes.data, 0
+        L.check(L.load().fwn_autotune(self._h(), ptr, qo.ctypes.data if qo is not None else None, n, fl,
+                                      cache_path.encode() if cache_path else None, C.byref(w), C.byref(m),
+                                      C.byref(ms)))
+        return w.value, m.value, ms.value
+
     def last_timing(self):
         tr, tot = C.c_double(), C.c_double()
         L.check(L.load().fwn_last_timing(self._h(), C.byref(tr), C.byref(tot)))

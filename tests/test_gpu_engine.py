@@ -290,3 +290,33 @@ def test_large_open_mesh_vs_gpu_bruteforce():
     wb = _wf(e.bruteforce_f64(sub))
     err = np.abs(w - wb)
     assert err.max() <= 4e-3 and err.mean() <= 1e-3, (err.max(), err.mean())
+
+
+def test_launch_shapes_give_identical_results():
+    """Every point's result is independent of the launch shape (grid / persistent, waves per block)."""
+    meshes = [W.displaced_sphere(20, 3, delete_fraction=0.05), W.torus(60, 20, seed=2, amplitude=0.2),
+              W.icosphere(4)]
+    pts = [W.uniform_queries(*m.bbox(), n, 70 + i, 1.3) for i, (m, n) in enumerate(zip(meshes, [5000, 3000, 700]))]
+    mm = MultiFastWindingNumber([(m.vertices, m.facets) for m in meshes])
+    ref = mm.solid_angle_batch(pts)
+    for waves in (1, 2, 4, 8):
+        for mode in (L.FWN_LAUNCH_GRID, L.FWN_LAUNCH_PERSISTENT):
+            mm.set_launch(waves, mode)
+            np.testing.assert_array_equal(mm.solid_angle_batch(pts), ref)
+    with pytest.raises(L.FWNError):
+        mm.set_launch(3, L.FWN_LAUNCH_GRID)
+
+
+def test_autotune_picks_valid_shape_and_caches(tmp_path):
+    V, F = W.displaced_sphere(24, 4, delete_fraction=0.05).f32()
+    q = W.uniform_queries(V.min(0), V.max(0), 20000, 80, 1.3)
+    e = FastWindingNumber((V, F))
+    ref = e.solid_angle_batch(q)
+    cache = str(tmp_path / "tune.txt")
+    w, mode, ms = e.autotune(q, cache_path=cache)
+    assert w in (1, 2, 4, 8) and mode in (0, 1) and ms > 0
+    np.testing.assert_array_equal(e.solid_angle_batch(q), ref)
+    assert open(cache).read().count("\n") == 1
+    e2 = FastWindingNumber((V, F))
+    assert e2.autotune(q, cache_path=cache)[:2] == (w, mode)  # read back from the cache
+    assert open(cache).read().count("\n") == 1
