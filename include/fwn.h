@@ -182,6 +182,67 @@ int fwn_autotune(fwn_engine* e, const float* Q, const int64_t* q_off, int64_t nQ
  * number of launches it took; measured with HIP events on the engine's stream. */
 int fwn_last_timing(const fwn_engine* e, double* traverse_ms, double* total_ms);
 
+/* ---- learned surrogate (no reference counterpart) ------------------------------------------
+ * A multi-resolution hash-grid encoding feeding a 4 x 128 ReLU MLP (BF16 MFMA, FP32 accumulate),
+ * trained on the GPU for the mesh of a single-mesh engine on winding numbers w = omega / (4 pi)
+ * that the exact engine generates.  Queries fall back to the exact engine when the prediction is
+ * within `band` of 0.5 or the point lies in a cell of the distance-bound grid (cells touched by a
+ * triangle's box grown by dist_rel x bbox diagonal). */
+typedef struct fwn_surrogate fwn_surrogate;
+
+typedef struct fwn_surrogate_options {
+    int32_t levels;        /* hash-grid levels (16) */
+    int32_t log2_table;    /* entries per level = 2^log2_table (19) */
+    int32_t base_res;      /* coarsest level resolution (16) */
+    int32_t max_res;       /* finest level resolution (2048) */
+    int32_t hidden_layers; /* 4 hidden layers of width 128 */
+    int32_t batch;         /* training batch (65536) */
+    int32_t steps;         /* Adam steps (3000) */
+    int32_t grid_res;      /* distance-bound grid cells per axis (1024) */
+    float lr;              /* Adam learning rate (5e-3; x0.1 decay over the last third) */
+    float band;            /* fallback band around w = 0.5 (0.2) */
+    float dist_rel;        /* distance bound as a fraction of the bbox diagonal (2e-3) */
+    float reserved_f;
+    int64_t n_samples;     /* training points labelled by the exact engine (20,000,000) */
+    uint64_t seed;
+} fwn_surrogate_options;
+
+typedef struct fwn_surrogate_info {
+    int32_t levels;
+    int32_t res[16];       /* per-level resolution */
+    int32_t dense[16];     /* 1 when the level is stored densely (no hashing) */
+    int64_t table_floats;  /* levels x 2^log2_table x 2 */
+    int64_t mlp_floats;    /* packed W0 b0 W1 b1 W2 b2 W3 b3 w4 b4 (row-major K x N weights) */
+    float delta;           /* absolute distance bound */
+    float last_loss;       /* mean squared error of the last training batch */
+    double train_ms;       /* sampling + labelling + training, device time */
+} fwn_surrogate_info;
+
+int fwn_surrogate_options_default(fwn_surrogate_options* opt);
+int fwn_surrogate_create(fwn_surrogate** out, fwn_engine* exact, const fwn_surrogate_options* opt);
+int fwn_surrogate_destroy(fwn_surrogate* s);
+int fwn_surrogate_train(fwn_surrogate* s);
+int fwn_surrogate_info_get(const fwn_surrogate* s, fwn_surrogate_info* info);
+
+/* Winding numbers w (= omega / (4 pi)) and is_inside of nQ points: surrogate where confident,
+ * exact engine for the fallback points (n_fallback_out: how many).  flags: FWN_DEVICE_POINTERS. */
+int fwn_surrogate_query(fwn_surrogate* s, const float* Q, int64_t nQ, float* w_out, uint8_t* inside_out,
+                        int32_t flags, int64_t* n_fallback_out);
+
+/* Raw network predictions and fallback flags (no exact evaluation). */
+int fwn_surrogate_predict(fwn_surrogate* s, const float* Q, int64_t nQ, float* pred_out, uint8_t* fallback_out,
+                          int32_t flags);
+
+/* Parameter access (host arrays, FP32 masters): copies out when *_out is not NULL, then in when
+ * *_in is not NULL. */
+int fwn_surrogate_params(fwn_surrogate* s, float* table_out, float* mlp_out, const float* table_in,
+                         const float* mlp_in);
+
+/* One forward + backward pass on n host points with targets y (no update): mean squared error,
+ * predictions, and the gradients of the table and the packed MLP (host arrays). */
+int fwn_surrogate_gradients(fwn_surrogate* s, const float* Q, const float* y, int64_t n, float* loss_out,
+                            float* pred_out, float* g_table_out, float* g_mlp_out);
+
 /* Thread-local message describing the last failure on this thread. */
 const char* fwn_last_error(void);
 

@@ -18,37 +18,7 @@ namespace fwn {
 
 namespace {
 thread_local std::string g_last_error;
-
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev)
-    {
-        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-        FWN_HIP(hipSetDevice(dev));
-    }
-    ~DeviceGuard()
-    {
-        if (prev >= 0) (void)hipSetDevice(prev);
-    }
-};
-
-template <class Body>
-int guarded(const char* fn, Body&& body)
-{
-    try {
-        body();
-        return FWN_OK;
-    } catch (const Failure& f) {
-        g_last_error = std::string(fn) + ": " + f.msg;
-        return f.status;
-    } catch (const std::bad_alloc&) {
-        g_last_error = std::string(fn) + ": host allocation failed";
-        return FWN_OUT_OF_MEMORY;
-    } catch (...) {
-        g_last_error = std::string(fn) + ": unknown failure";
-        return FWN_HIP_ERROR;
-    }
-}
+} // namespace
 
 void check_options(const fwn_options& o)
 {
@@ -66,6 +36,8 @@ void check_options(const fwn_options& o)
     if (o.tree_width != 2 && o.tree_width != 4 && o.tree_width != 8)
         fail(FWN_INVALID_ARGUMENT, "tree_width must be 2, 4 or 8");
 }
+
+namespace {
 
 Engine* make_engine(const fwn_options* opt)
 {
@@ -134,6 +106,7 @@ const Engine* E(const fwn_engine* h)
 } // namespace
 
 void set_error(const std::string& msg) { g_last_error = msg; }
+const char* last_error_text() { return g_last_error.c_str(); }
 
 void fail(int status, const std::string& msg) { throw Failure{status, msg}; }
 

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <mutex>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -114,6 +115,42 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
 void run_bruteforce(Engine& e, const float* Q, int64_t nQ, double* out, bool device_ptrs);
 
 int grid_blocks(int64_t n, int threads);
+
+// validates an options struct (throws Failure)
+void check_options(const fwn_options& o);
+
+// Sets the current device for a scope and restores the previous one.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        FWN_HIP(hipSetDevice(dev));
+    }
+    ~DeviceGuard()
+    {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+// Runs body, turning failures into a status code and the thread-local error message.
+template <class Body>
+int guarded(const char* fn, Body&& body)
+{
+    try {
+        body();
+        return FWN_OK;
+    } catch (const Failure& f) {
+        set_error(std::string(fn) + ": " + f.msg);
+        return f.status;
+    } catch (const std::bad_alloc&) {
+        set_error(std::string(fn) + ": host allocation failed");
+        return FWN_OUT_OF_MEMORY;
+    } catch (...) {
+        set_error(std::string(fn) + ": unknown failure");
+        return FWN_HIP_ERROR;
+    }
+}
 
 } // namespace fwn
 
