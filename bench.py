@@ -164,6 +164,71 @@ def bench_near_surface(steps: int = 5):
     }
 
 
+def bench_surrogate(mesh, steps: int = 3):
+    """Surrogate on the ~1M-triangle mesh: 16 x 2^19 hash grid (2 FP16 features) + 4 x 128 ReLU MLP
+    (BF16 MFMA), trained on 20M points labelled by the exact engine (half uniform, half near the
+    surface); 67,108,864 uniform points in the box answered by surrogate + exact fallback
+    (prediction within 0.2 of 0.5, or inside the distance bound), against the exact engine."""
+    import numpy as np
+    import torch
+
+    from lagrange_amd import workloads as W
+    from lagrange_amd.surrogate import Surrogate
+    from lagrange_amd.winding import FastWindingNumber
+
+    V, F = mesh.f32()
+    e = FastWindingNumber((V, F))
+    s = Surrogate(e)
+    info = s.train()
+    lo, hi = V.min(0), V.max(0)
+    n = 1 << 26
+    q = W.uniform_queries(lo, hi, n, 5, 1.0)
+    qd = torch.from_numpy(q).cuda()
+    del q
+    om, ins_e = e.solid_angle_batch(qd, with_inside=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        om, ins_e = e.solid_angle_batch(qd, with_inside=True)
+    torch.cuda.synchronize()
+    t_exact = (time.perf_counter() - t0) / steps
+    w, ins_s, nf = s.query(qd)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        w, ins_s, nf = s.query(qd)
+    torch.cuda.synchronize()
+    t_sur = (time.perf_counter() - t0) / steps
+    si = s.info()
+    mis = int((ins_s != ins_e).sum().item())
+    # FP64 check of both on a strided sample
+    idx = torch.arange(0, n, n // 2048, device=qd.device)
+    wb = e.bruteforce_f64(qd[idx]).cpu().numpy() / (4 * np.pi)
+    amb = np.abs(wb - 0.5) <= 1e-3
+    mis64 = int(((ins_s[idx].cpu().numpy() != (wb > 0.5)) & ~amb).sum())
+    mis64_e = int(((ins_e[idx].cpu().numpy() != (wb > 0.5)) & ~amb).sum())
+    mlp_flops = n * 2 * (32 * 128 + 3 * 128 * 128)
+    return {
+        "workload": f"{F.shape[0]} triangles, {n} points uniform in the bbox; 16 levels x 2^19 x 2 FP16 hash grid, "
+                    "MLP 32-128-128-128-128-1 ReLU, 20M training points labelled by the exact engine",
+        "exact_queries_per_s": n / t_exact,
+        "surrogate_queries_per_s": n / t_sur,
+        "surrogate_vs_exact": t_exact / t_sur,
+        "fallback_fraction": nf / n,
+        "misclassified_vs_exact": mis / n,
+        "misclassified_vs_fp64_sample": mis64,
+        "exact_misclassified_vs_fp64_sample": mis64_e,
+        "fp64_sample": int(idx.numel()),
+        "train_s": si["train_ms"] / 1e3,
+        "train_loss": si["last_loss"],
+        "encode_ms": si["encode_ms"],
+        "mlp_ms": si["mlp_ms"],
+        "fallback_ms": si["fallback_ms"],
+        "mfma": {"kernel": "k_sg_mlp", "achieved": mlp_flops / (si["mlp_ms"] * 1e-3) / 1e12, "peak": 2500.0,
+                 "unit": "TFLOP/s", "frac": mlp_flops / (si["mlp_ms"] * 1e-3) / 1e12 / 2500.0},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,8 +238,8 @@ def main():
     ap.add_argument("--leaf-max", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=4)
-    ap.add_argument("--extras", default="batched,near",
-                    help="comma list of extra workloads reported beside the headline: batched, near (or none)")
+    ap.add_argument("--extras", default="batched,near,surrogate",
+                    help="comma list of extra workloads reported beside the headline: batched, near, surrogate")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (scale for rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -292,6 +357,10 @@ def main():
         line["batched"] = bench_batched()
     if "near" in extras:
         line["near_surface"] = bench_near_surface()
+    if "surrogate" in extras:
+        del qd, om, ins
+        torch.cuda.empty_cache()
+        line["surrogate"] = bench_surrogate(mesh)
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
         threads = min(threads, 16)

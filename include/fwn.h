@@ -216,9 +216,14 @@ typedef struct fwn_surrogate_info {
     float delta;           /* absolute distance bound */
     float last_loss;       /* mean squared error of the last training batch */
     double train_ms;       /* sampling + labelling + training, device time */
+    double encode_ms;      /* last query / predict call: hash-grid encoding kernel */
+    double mlp_ms;         /* last query / predict call: fused MLP kernel */
+    double fallback_ms;    /* last query call: compaction + exact engine on the fallback points */
+    double query_ms;       /* last query call: whole call on the device */
 } fwn_surrogate_info;
 
 int fwn_surrogate_options_default(fwn_surrogate_options* opt);
+/* The surrogate keeps a pointer to `exact`, which must outlive it (destroy the surrogate first). */
 int fwn_surrogate_create(fwn_surrogate** out, fwn_engine* exact, const fwn_surrogate_options* opt);
 int fwn_surrogate_destroy(fwn_surrogate* s);
 int fwn_surrogate_train(fwn_surrogate* s);

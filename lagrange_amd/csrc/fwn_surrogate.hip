@@ -526,7 +526,8 @@ __global__ __launch_bounds__(IW * 64) __attribute__((amdgpu_waves_per_eu(1, 1)))
 // host object
 // ============================================================================================
 struct Surrogate {
-    Engine* exact = nullptr;
+    Engine* exact = nullptr; // not owned; must outlive the surrogate (fwn.h)
+    int device = 0;
     fwn_surrogate_options opt{};
     GridParams grid{};
     int64_t n_table = 0;  // floats in the table (L * T * 2)
@@ -546,6 +547,8 @@ struct Surrogate {
     DBuf P, y, X0, Xh[4], dP, dQ, dX0, pred, dpred, parts, loss, fb;
     int adam_t = 0;
     int n_cu = 0;
+    hipEvent_t ev[6] = {};
+    double encode_ms = 0.0, mlp_ms = 0.0, fallback_ms = 0.0, query_ms = 0.0;
 };
 
 namespace {
@@ -695,6 +698,7 @@ void surrogate_init(Surrogate& S, Engine* exact, const fwn_surrogate_options& o)
     sg_check(o);
     if (exact->n_meshes != 1) fail(FWN_INVALID_ARGUMENT, "the surrogate is built on a single-mesh engine");
     S.exact = exact;
+    S.device = exact->device;
     S.opt = o;
     hipStream_t s = exact->stream;
     GridParams& g = S.grid;
@@ -869,9 +873,13 @@ void surrogate_predict(Surrogate& S, const float* P, int64_t n, float* pred_out,
         S.fb.reserve(n);
         fb = S.fb.as<uint8_t>();
     }
+    if (!S.ev[0])
+        for (auto& ev : S.ev) FWN_HIP(hipEventCreate(&ev));
+    FWN_HIP(hipEventRecord(S.ev[0], s));
     hipLaunchKernelGGL(k_sg_encode, dim3(grid_blocks(n, 256)), dim3(256), 0, s, P, n, S.grid, S.table_h.as<__half>(),
                        S.R, S.bits.as<uint32_t>(), S.X0.as<uint16_t>(), fb);
     FWN_LAUNCH_CHECK("k_sg_encode");
+    FWN_HIP(hipEventRecord(S.ev[1], s));
     if (S.n_cu == 0) {
         hipDeviceProp_t prop;
         FWN_HIP(hipGetDeviceProperties(&prop, S.exact->device));
@@ -883,6 +891,13 @@ void surrogate_predict(Surrogate& S, const float* P, int64_t n, float* pred_out,
                        S.X0.as<uint16_t>(), n, S.frags.as<uint16_t>(), S.mlp.as<float>(), S.off_b[0], S.off_b[1],
                        S.off_b[2], S.off_b[3], S.off_w[4], S.off_b[4], S.opt.band, pred_out, fb);
     FWN_LAUNCH_CHECK("k_sg_mlp");
+    FWN_HIP(hipEventRecord(S.ev[2], s));
+    FWN_HIP(hipEventSynchronize(S.ev[2]));
+    float a = 0.0f, b = 0.0f;
+    FWN_HIP(hipEventElapsedTime(&a, S.ev[0], S.ev[1]));
+    FWN_HIP(hipEventElapsedTime(&b, S.ev[1], S.ev[2]));
+    S.encode_ms = a;
+    S.mlp_ms = b;
 }
 
 } // namespace fwn
@@ -958,7 +973,13 @@ void surrogate_query(Surrogate& S, SurrogateQueryScratch& Q, const float* P, int
     Q.fb.reserve(n);
     Q.cnt.reserve(sizeof(int32_t) * n);
     Q.pos.reserve(sizeof(int32_t) * (n + 1));
+    hipEvent_t q0, q1, q2;
+    FWN_HIP(hipEventCreate(&q0));
+    FWN_HIP(hipEventCreate(&q1));
+    FWN_HIP(hipEventCreate(&q2));
+    FWN_HIP(hipEventRecord(q0, s));
     surrogate_predict(S, dP, n, Q.pred.as<float>(), Q.fb.as<uint8_t>());
+    FWN_HIP(hipEventRecord(q1, s));
     hipLaunchKernelGGL(k_flags_i32, dim3(grid_blocks(n, 256)), dim3(256), 0, s, Q.fb.as<uint8_t>(), n, Q.cnt.as<int32_t>());
     exclusive_scan_i32(Q.cnt.as<int32_t>(), Q.pos.as<int32_t>(), n, Q.scan_tmp, s);
     int32_t nf = 0;
@@ -975,11 +996,20 @@ void surrogate_query(Surrogate& S, SurrogateQueryScratch& Q, const float* P, int
     hipLaunchKernelGGL(k_merge, dim3(grid_blocks(n, 256)), dim3(256), 0, s, Q.pred.as<float>(), Q.fb.as<uint8_t>(),
                        Q.pos.as<int32_t>(), Q.omf.as<float>(), n, dW, dIn);
     FWN_LAUNCH_CHECK("k_merge");
+    FWN_HIP(hipEventRecord(q2, s));
     if (!dev) {
         FWN_HIP(hipMemcpyAsync(w, dW, sizeof(float) * n, hipMemcpyDeviceToHost, s));
         if (inside) FWN_HIP(hipMemcpyAsync(inside, dIn, n, hipMemcpyDeviceToHost, s));
     }
     FWN_HIP(hipStreamSynchronize(s));
+    float fa = 0.0f, fq = 0.0f;
+    FWN_HIP(hipEventElapsedTime(&fa, q1, q2));
+    FWN_HIP(hipEventElapsedTime(&fq, q0, q2));
+    S.fallback_ms = fa;
+    S.query_ms = fq;
+    FWN_HIP(hipEventDestroy(q0));
+    FWN_HIP(hipEventDestroy(q1));
+    FWN_HIP(hipEventDestroy(q2));
     if (n_fallback) *n_fallback = nf;
 }
 
@@ -1046,8 +1076,11 @@ int fwn_surrogate_destroy(fwn_surrogate* h)
 {
     return fwn::guarded("fwn_surrogate_destroy", [&] {
         if (!h) return;
-        (void)hipSetDevice(h->S.exact->device);
-        (void)hipStreamSynchronize(h->S.exact->stream);
+        // only the device id is used here: the engine may already be gone
+        (void)hipSetDevice(h->S.device);
+        (void)hipDeviceSynchronize();
+        for (auto& ev : h->S.ev)
+            if (ev) (void)hipEventDestroy(ev);
         delete h;
     });
 }
@@ -1078,6 +1111,10 @@ int fwn_surrogate_info_get(const fwn_surrogate* h, fwn_surrogate_info* info)
         info->delta = S.delta;
         info->last_loss = S.last_loss;
         info->train_ms = S.train_ms;
+        info->encode_ms = S.encode_ms;
+        info->mlp_ms = S.mlp_ms;
+        info->fallback_ms = S.fallback_ms;
+        info->query_ms = S.query_ms;
     });
 }
 

@@ -28,7 +28,8 @@ class SurrogateInfo(C.Structure):
     _fields_ = [
         ("levels", C.c_int32), ("res", C.c_int32 * 16), ("dense", C.c_int32 * 16),
         ("table_floats", C.c_int64), ("mlp_floats", C.c_int64), ("delta", C.c_float),
-        ("last_loss", C.c_float), ("train_ms", C.c_double),
+        ("last_loss", C.c_float), ("train_ms", C.c_double), ("encode_ms", C.c_double), ("mlp_ms", C.c_double),
+        ("fallback_ms", C.c_double), ("query_ms", C.c_double),
     ]
 
 
@@ -73,6 +74,9 @@ class Surrogate:
         L.check(lib.fwn_surrogate_create(C.byref(h), engine._h(), C.byref(o)))
         self._h = h
         self.options = o
+        engine._add_dependent(self)
+
+    _h = None
 
     def close(self):
         if self._h:
@@ -94,7 +98,8 @@ class Surrogate:
         L.check(_bind().fwn_surrogate_info_get(self._h, C.byref(i)))
         return {"levels": i.levels, "res": list(i.res)[: i.levels], "dense": list(i.dense)[: i.levels],
                 "table_floats": i.table_floats, "mlp_floats": i.mlp_floats, "delta": i.delta,
-                "last_loss": i.last_loss, "train_ms": i.train_ms}
+                "last_loss": i.last_loss, "train_ms": i.train_ms, "encode_ms": i.encode_ms, "mlp_ms": i.mlp_ms,
+                "fallback_ms": i.fallback_ms, "query_ms": i.query_ms}
 
     def query(self, points):
         """(w, inside, n_fallback): numpy in -> numpy out, torch device tensor -> device tensors."""

