@@ -167,7 +167,7 @@ def bench_near_surface(steps: int = 5):
 def bench_surrogate(mesh, steps: int = 3):
     """Surrogate on the ~1M-triangle mesh: 16 x 2^19 hash grid (2 FP16 features) + 4 x 128 ReLU MLP
     (BF16 MFMA), trained on 20M points labelled by the exact engine (half uniform, half near the
-    surface); 67,108,864 uniform points in the box answered by surrogate + exact fallback
+    surface; points inside the distance bound carry no loss); 67,108,864 uniform points in the box answered by surrogate + exact fallback
     (prediction within 0.2 of 0.5, or inside the distance bound), against the exact engine."""
     import numpy as np
     import torch

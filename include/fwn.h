@@ -202,7 +202,7 @@ typedef struct fwn_surrogate_options {
     float lr;              /* Adam learning rate (5e-3; x0.1 decay over the last third) */
     float band;            /* fallback band around w = 0.5 (0.2) */
     float dist_rel;        /* distance bound as a fraction of the bbox diagonal (2e-3) */
-    float reserved_f;
+    float near_frac;       /* fraction of training points drawn near the surface (0.5) */
     int64_t n_samples;     /* training points labelled by the exact engine (20,000,000) */
     uint64_t seed;
 } fwn_surrogate_options;

@@ -3,7 +3,8 @@
 // w(q) ~ MLP(hash_grid(q)): a multi-resolution hash-grid encoding (16 levels, 2^19 entries per
 // level, 2 FP16 features) feeds a 32 -> 128 -> 128 -> 128 -> 128 -> 1 ReLU MLP whose hidden layers
 // run on BF16 MFMA with FP32 accumulation.  It is trained per mesh on winding numbers generated
-// by the exact engine (half uniform in the box, half near the surface) with Adam, all on the GPU:
+// by the exact engine (uniform in the box and near the surface) with Adam, all on the GPU; points
+// inside the distance bound, which are always answered exactly, carry no loss:
 //
 //   k_sample, exact engine      training points and labels
 //   k_encode_fwd / k_encode_bwd hash-grid lookup (trilinear over 8 corners) and its scatter-add
@@ -207,15 +208,33 @@ __global__ __launch_bounds__(256) void k_out_fwd(const uint16_t* __restrict__ X4
     pred[i] = s + b4[0];
 }
 
+__device__ __forceinline__ bool near_cell(const uint32_t* __restrict__ bits, int R, float px, float py, float pz)
+{
+    // points outside the grid box count as near (the network was not trained there)
+    if (!(px >= 0.0f && px <= 1.0f && py >= 0.0f && py <= 1.0f && pz >= 0.0f && pz <= 1.0f)) return true;
+    const int x = min(R - 1, (int)(px * (float)R)), y = min(R - 1, (int)(py * (float)R)),
+              z = min(R - 1, (int)(pz * (float)R));
+    const uint64_t cell = ((uint64_t)z * R + y) * R + x;
+    return (bits[cell >> 5] >> (cell & 31u)) & 1u;
+}
+
+// Mean squared error and its gradient.  With `bits` set, points in marked cells (always answered
+// by the exact engine) carry no loss: the network's capacity goes to the region it serves.
 __global__ __launch_bounds__(256) void k_loss_grad(const float* __restrict__ pred, const float* __restrict__ y,
-                                                   int64_t n, float scale, float* __restrict__ dpred,
-                                                   float* __restrict__ loss)
+                                                   int64_t n, float scale, const float* __restrict__ P, GridParams g,
+                                                   int R, const uint32_t* __restrict__ bits,
+                                                   float* __restrict__ dpred, float* __restrict__ loss)
 {
     __shared__ float part[256];
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     float e = 0.0f;
     if (i < n) {
         e = pred[i] - y[i];
+        if (bits) {
+            const float px = (P[3 * i] - g.lo[0]) * g.inv[0], py = (P[3 * i + 1] - g.lo[1]) * g.inv[1],
+                        pz = (P[3 * i + 2] - g.lo[2]) * g.inv[2];
+            if (near_cell(bits, R, px, py, pz)) e = 0.0f;
+        }
         dpred[i] = 2.0f * e * scale;
     }
     part[threadIdx.x] = e * e;
@@ -273,17 +292,18 @@ __global__ void k_cast(const float* __restrict__ p, int64_t n, uint16_t* __restr
     if (h_out) h_out[i] = __float2half(p[i]);
 }
 
-// Training points: even index uniform in the box, odd index near the surface (a uniformly chosen
-// triangle, a uniform barycentric point, offset +-diag * 10^U(-7, -2) along its normal).
+// Training points: with probability near_frac a point near the surface (a uniformly chosen
+// triangle, a uniform barycentric point, offset +-diag * 10^U(lo_exp, -1) along its normal),
+// otherwise uniform in the box.
 __global__ __launch_bounds__(256) void k_sample(int64_t n, uint64_t seed, int64_t offset, const float* __restrict__ box,
-                                                const uint32_t* __restrict__ tris, int64_t ntris,
-                                                float* __restrict__ P)
+                                                const uint32_t* __restrict__ tris, int64_t ntris, float near_frac,
+                                                float lo_exp, float* __restrict__ P)
 {
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (j >= n) return;
     const uint64_t i = (uint64_t)(j + offset);
     const float lo[3] = {box[0], box[1], box[2]}, hi[3] = {box[3], box[4], box[5]};
-    if ((i & 1ull) == 0ull || ntris == 0) {
+    if (ntris == 0 || rng_unit(seed, 9, i) >= near_frac) {
         for (int d = 0; d < 3; ++d) P[3 * j + d] = lo[d] + (hi[d] - lo[d]) * rng_unit(seed, 10 + d, i);
         return;
     }
@@ -298,7 +318,7 @@ __global__ __launch_bounds__(256) void k_sample(int64_t n, uint64_t seed, int64_
     const float inv = len > 0.0f ? 1.0f / len : 0.0f;
     float diag2 = 0.0f;
     for (int d = 0; d < 3; ++d) diag2 += (hi[d] - lo[d]) * (hi[d] - lo[d]);
-    const float off = sqrtf(diag2) * exp10f(-7.0f + 5.0f * u3) * (u4 < 0.5f ? -1.0f : 1.0f);
+    const float off = sqrtf(diag2) * exp10f(lo_exp + (-1.0f - lo_exp) * u3) * (u4 < 0.5f ? -1.0f : 1.0f);
     for (int d = 0; d < 3; ++d) P[3 * j + d] = a * v[d] + b * v[3 + d] + c * v[6 + d] + off * nn[d] * inv;
 }
 
@@ -328,16 +348,6 @@ __global__ __launch_bounds__(256) void k_mark_near(const uint32_t* __restrict__ 
                 const uint64_t cell = ((uint64_t)z * R + y) * R + x;
                 atomicOr(&bits[cell >> 5], 1u << (cell & 31u));
             }
-}
-
-__device__ __forceinline__ bool near_cell(const uint32_t* __restrict__ bits, int R, float px, float py, float pz)
-{
-    // points outside the grid box count as near (the network was not trained there)
-    if (!(px >= 0.0f && px <= 1.0f && py >= 0.0f && py <= 1.0f && pz >= 0.0f && pz <= 1.0f)) return true;
-    const int x = min(R - 1, (int)(px * (float)R)), y = min(R - 1, (int)(py * (float)R)),
-              z = min(R - 1, (int)(pz * (float)R));
-    const uint64_t cell = ((uint64_t)z * R + y) * R + x;
-    return (bits[cell >> 5] >> (cell & 31u)) & 1u;
 }
 
 // fallback flag per point: prediction within band of 0.5, or a marked cell
@@ -563,6 +573,8 @@ void sg_check(const fwn_surrogate_options& o)
     if (o.steps < 0 || o.n_samples < o.batch) fail(FWN_INVALID_ARGUMENT, "need steps >= 0 and n_samples >= batch");
     if (!(o.lr > 0.0f) || !(o.band >= 0.0f) || !(o.band < 0.5f)) fail(FWN_INVALID_ARGUMENT, "bad lr or band");
     if (o.grid_res < 8 || o.grid_res > 2048) fail(FWN_INVALID_ARGUMENT, "grid_res must be in [8, 2048]");
+    if (!(o.dist_rel >= 0.0f) || !(o.near_frac >= 0.0f && o.near_frac <= 1.0f))
+        fail(FWN_INVALID_ARGUMENT, "need dist_rel >= 0 and near_frac in [0, 1]");
 }
 
 inline int gb(int64_t n, int t) { return grid_blocks(n, t); }
@@ -618,7 +630,8 @@ void layer_wgrad(Surrogate& S, hipStream_t s, const uint16_t* X, int K, const ui
 
 // one forward + backward over n points (targets y), gradients accumulated into g_table / g_mlp;
 // the MLP gradient buffer is overwritten for the weights (reduced partials) and added for biases.
-float forward_backward(Surrogate& S, hipStream_t s, const float* P, const float* y, int64_t n)
+// With mask_near, points in cells of the distance-bound grid carry no loss.
+float forward_backward(Surrogate& S, hipStream_t s, const float* P, const float* y, int64_t n, bool mask_near)
 {
     forward(S, s, P, n, false);
     S.dpred.reserve(sizeof(float) * n);
@@ -627,7 +640,8 @@ float forward_backward(Surrogate& S, hipStream_t s, const float* P, const float*
     S.dQ.reserve(sizeof(uint16_t) * kHidden * n);
     S.dX0.reserve(sizeof(float) * 32 * n);
     FWN_HIP(hipMemsetAsync(S.loss.p, 0, sizeof(float), s));
-    hipLaunchKernelGGL(k_loss_grad, dim3(gb(n, 256)), dim3(256), 0, s, S.pred.as<float>(), y, n, 1.0f / (float)n,
+    hipLaunchKernelGGL(k_loss_grad, dim3(gb(n, 256)), dim3(256), 0, s, S.pred.as<float>(), y, n, 1.0f / (float)n, P,
+                       S.grid, S.R, mask_near ? S.bits.as<uint32_t>() : (const uint32_t*)nullptr,
                        S.dpred.as<float>(), S.loss.as<float>());
     FWN_LAUNCH_CHECK("k_loss_grad");
     const int64_t rows = 512;
@@ -807,7 +821,7 @@ float surrogate_gradients(Surrogate& S, const float* P, const float* y, int64_t 
     hipStream_t s = S.exact->stream;
     zero(S.g_table, sizeof(float) * S.n_table, s);
     zero(S.g_mlp, sizeof(float) * S.n_mlp, s);
-    return forward_backward(S, s, P, y, n);
+    return forward_backward(S, s, P, y, n, false);
 }
 
 void surrogate_train(Surrogate& S)
@@ -830,8 +844,10 @@ void surrogate_train(Surrogate& S)
     DBuf dbox;
     dbox.alloc(sizeof(box));
     FWN_HIP(hipMemcpyAsync(dbox.p, box, sizeof(box), hipMemcpyHostToDevice, s));
+    // near-surface offsets start at half the distance bound: closer points are answered exactly
+    const float lo_exp = std::min(-1.5f, std::log10(std::max(S.opt.dist_rel, 1e-7f)) - 0.3f);
     hipLaunchKernelGGL(k_sample, dim3(gb(ns, 256)), dim3(256), 0, s, ns, S.opt.seed, (int64_t)0, dbox.as<float>(),
-                       E.tris.as<uint32_t>(), E.n_tris, S.P.as<float>());
+                       E.tris.as<uint32_t>(), E.n_tris, S.opt.near_frac, lo_exp, S.P.as<float>());
     FWN_LAUNCH_CHECK("k_sample");
     // labels from the exact engine: w = omega / (4 pi)
     run_queries(E, S.P.as<float>(), nullptr, ns, S.y.as<float>(), nullptr, nullptr, true);
@@ -846,7 +862,7 @@ void surrogate_train(Surrogate& S)
         // learning rate: constant, then x0.1 decay over the last third
         const float frac = (float)step / (float)std::max(1, S.opt.steps);
         const float lr = S.opt.lr * (frac < 2.0f / 3.0f ? 1.0f : std::pow(0.1f, (frac - 2.0f / 3.0f) * 3.0f));
-        loss = forward_backward(S, s, S.P.as<float>() + 3 * b * B, S.y.as<float>() + b * B, B);
+        loss = forward_backward(S, s, S.P.as<float>() + 3 * b * B, S.y.as<float>() + b * B, B, true);
         adam_step(S, s, lr);
     }
     FWN_HIP(hipEventRecord(t1, s));
@@ -1050,6 +1066,7 @@ int fwn_surrogate_options_default(fwn_surrogate_options* o)
         o->lr = 5e-3f;
         o->band = 0.2f;
         o->dist_rel = 2e-3f;
+        o->near_frac = 0.5f;
         o->n_samples = 20000000;
         o->seed = 5;
     });

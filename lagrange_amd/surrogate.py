@@ -19,7 +19,7 @@ class SurrogateOptions(C.Structure):
     _fields_ = [
         ("levels", C.c_int32), ("log2_table", C.c_int32), ("base_res", C.c_int32), ("max_res", C.c_int32),
         ("hidden_layers", C.c_int32), ("batch", C.c_int32), ("steps", C.c_int32), ("grid_res", C.c_int32),
-        ("lr", C.c_float), ("band", C.c_float), ("dist_rel", C.c_float), ("reserved_f", C.c_float),
+        ("lr", C.c_float), ("band", C.c_float), ("dist_rel", C.c_float), ("near_frac", C.c_float),
         ("n_samples", C.c_int64), ("seed", C.c_uint64),
     ]
 
