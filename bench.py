@@ -164,6 +164,77 @@ def bench_near_surface(steps: int = 5):
     }
 
 
+def bench_voxel(mesh, steps: int = 5):
+    """Winding-number voxel signs (mesh_to_volume's interior test, batched) on the ~1M-triangle
+    mesh: every voxel of the index box around the mesh (bbox + 3-voxel band) at voxel size
+    0.002 x bbox diagonal, centres generated on the GPU, flags left in HBM.  Beside it, the same
+    voxels sent as host points (centres made on the host, uploaded, flags copied back)."""
+    import numpy as np
+    import torch
+
+    from lagrange_amd import volume as VOL
+    from lagrange_amd.winding import FastWindingNumber
+
+    V, F = mesh.f32()
+    e = FastWindingNumber((V, F))
+    vs = VOL.resolve_voxel_size(V, -0.002)
+    lo, dims = VOL.index_box(V, vs)
+    n = int(np.prod(dims.astype(np.int64)))
+    dev = torch.device("cuda:0")
+    ins = e.sign_voxel_box(lo, dims, vs, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ins = e.sign_voxel_box(lo, dims, vs, device=dev)
+    torch.cuda.synchronize()
+    t_dev = (time.perf_counter() - t0) / steps
+    # host-points path, timed once (centre generation on the host included)
+    z, y, x = np.meshgrid(np.arange(dims[2]), np.arange(dims[1]), np.arange(dims[0]), indexing="ij")
+    t0 = time.perf_counter()
+    ijk = np.stack([x.ravel() + lo[0], y.ravel() + lo[1], z.ravel() + lo[2]], axis=1)
+    c = VOL.voxel_centres(ijk, vs)
+    ins_h = e.is_inside_batch(c)
+    t_host = time.perf_counter() - t0
+    same = bool(np.array_equal(ins.cpu().numpy().ravel(), ins_h))
+    return {
+        "workload": f"{F.shape[0]} triangles, voxel box {int(dims[0])}x{int(dims[1])}x{int(dims[2])} "
+                    f"({n} voxels, voxel size {vs:.5g} = 0.002 x bbox diagonal)",
+        "voxels_per_s": n / t_dev,
+        "ms_per_box": t_dev * 1e3,
+        "host_points_path_voxels_per_s": n / t_host,
+        "inside_fraction": float(ins.float().mean().item()),
+        "identical_to_host_points_path": same,
+    }
+
+
+def bench_io(mesh):
+    """Host-side mesh input: the ~1M-triangle mesh written as OBJ (text) and binary PLY to a
+    temporary directory, then read back by the native reader (best of 3 loads each)."""
+    import tempfile
+
+    import numpy as np
+
+    from lagrange_amd import io as mio
+
+    V = mesh.vertices.astype(np.float64)
+    F = mesh.facets.astype(np.int32)
+    out = {"workload": f"{F.shape[0]} triangles, {V.shape[0]} vertices"}
+    with tempfile.TemporaryDirectory() as d:
+        for ext, key in ((".obj", "obj"), (".ply", "ply_binary")):
+            p = os.path.join(d, "mesh" + ext)
+            mio.save_mesh(p, V, F)
+            best = float("inf")
+            for _ in range(3):
+                t0 = time.perf_counter()
+                m = mio.load_mesh(p)
+                best = min(best, time.perf_counter() - t0)
+            same = bool(np.array_equal(m.vertices, V) and np.array_equal(m.facets, F))
+            size = os.path.getsize(p)
+            out[key] = {"load_s": best, "MB_per_s": size / best / 1e6, "triangles_per_s": F.shape[0] / best,
+                        "file_MB": size / 1e6, "round_trip_exact": same}
+    return out
+
+
 def bench_surrogate(mesh, steps: int = 3):
     """Surrogate on the ~1M-triangle mesh: 16 x 2^19 hash grid (2 FP16 features) + 4 x 128 ReLU MLP
     (BF16 MFMA), trained on 20M points labelled by the exact engine (half uniform, half near the
@@ -238,8 +309,9 @@ def main():
     ap.add_argument("--leaf-max", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=4)
-    ap.add_argument("--extras", default="batched,near,surrogate",
-                    help="comma list of extra workloads reported beside the headline: batched, near, surrogate")
+    ap.add_argument("--extras", default="batched,near,voxel,io,surrogate",
+                    help="comma list of extra workloads reported beside the headline: batched, near, voxel, "
+                         "io, surrogate")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (scale for rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -357,6 +429,10 @@ def main():
         line["batched"] = bench_batched()
     if "near" in extras:
         line["near_surface"] = bench_near_surface()
+    if "voxel" in extras:
+        line["voxel"] = bench_voxel(mesh)
+    if "io" in extras:
+        line["io"] = bench_io(mesh)
     if "surrogate" in extras:
         del qd, om, ins
         torch.cuda.empty_cache()

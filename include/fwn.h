@@ -4,18 +4,21 @@
  * This is the drop-in boundary under lagrange::winding::FastWindingNumber
  * (modules/winding/include/lagrange/winding/FastWindingNumber.h:24-108).  The reference
  * class owns a float/int copy of the mesh and a HDK_Sample::UT_SolidAngle<float,float>
- * engine (modules/winding/src/FastWindingNumber.cpp:139-196); here the same role is played
+ * engine (modules/winding/src/FastWindingNumber.cpp:28-85); here the same role is played
  * by an opaque fwn_engine holding a GPU-built tree in HBM.  All functions are plain C:
  * pointers and sizes, status codes, no exceptions cross the boundary.
  *
  * Entry points and the reference interface each replaces:
  *   fwn_create            <- FastWindingNumber(const SurfaceMesh&) / Impl::initialize + UT_SolidAngle::init
- *                            (FastWindingNumber.cpp:142-169, 198-209)
- *   fwn_destroy           <- ~FastWindingNumber (FastWindingNumber.cpp:212)
- *   fwn_solid_angle_point <- FastWindingNumber::solid_angle (FastWindingNumber.cpp:180-187, 221-224)
- *   fwn_is_inside_point   <- FastWindingNumber::is_inside (FastWindingNumber.cpp:171-178, 216-219)
+ *                            (FastWindingNumber.cpp:31-58, 87-98)
+ *   fwn_destroy           <- ~FastWindingNumber (FastWindingNumber.cpp:101)
+ *   fwn_solid_angle_point <- FastWindingNumber::solid_angle (FastWindingNumber.cpp:69-76, 110-113)
+ *   fwn_is_inside_point   <- FastWindingNumber::is_inside (FastWindingNumber.cpp:60-67, 105-108)
  *   fwn_solid_angle       <- the per-point calls above issued over an array of points, as
  *                            volume::mesh_to_volume does per voxel (modules/volume/src/mesh_to_volume.cpp:167-183)
+ *   fwn_sign_voxel_box /
+ *   fwn_sign_voxels       <- the interior-test lambda of volume::mesh_to_volume with Sign::WindingNumber
+ *                            (modules/volume/src/mesh_to_volume.cpp:175-183), voxel centres made on the GPU
  *   fwn_create_multi /
  *   fwn_solid_angle_multi <- no single reference call: many FastWindingNumber objects queried at once
  *   fwn_export_tree, fwn_query_stats, fwn_bruteforce_f64 -- measurement and verification hooks
@@ -121,7 +124,7 @@ enum { FWN_STAT_TESTS = 0, FWN_STAT_FAR = 1, FWN_STAT_TRI32 = 2, FWN_STAT_TRI64 
 int fwn_options_default(fwn_options* opt);
 
 /* One mesh.  V: nV x 3 float (row-major), F: nF x 3 int32, host pointers.  The engine copies
- * what it needs; the caller's arrays may be freed afterwards (FastWindingNumber.cpp:150-168). */
+ * what it needs; the caller's arrays may be freed afterwards (FastWindingNumber.cpp:39-57). */
 int fwn_create(fwn_engine** out, const float* V, int64_t nV, const int32_t* F, int64_t nF,
                const fwn_options* opt);
 
@@ -136,13 +139,29 @@ int fwn_destroy(fwn_engine* e);
 /* ---- queries -------------------------------------------------------------------------- */
 /* Solid angle (steradians, as FastWindingNumber::solid_angle) of nQ points Q (nQ x 3 float).
  * inside_out (nullable) receives (omega / (4 pi) > 0.5) evaluated in double, byte per point
- * (FastWindingNumber.cpp:177).  flags: FWN_DEVICE_POINTERS. */
+ * (FastWindingNumber.cpp:66).  flags: FWN_DEVICE_POINTERS. */
 int fwn_solid_angle(fwn_engine* e, const float* Q, int64_t nQ, float* omega_out, uint8_t* inside_out,
                     int32_t flags);
 
 /* Points of every mesh in one launch: points of mesh m are [q_off[m], q_off[m+1]) of Q. */
 int fwn_solid_angle_multi(fwn_engine* e, const float* Q, const int64_t* q_off, float* omega_out,
                           uint8_t* inside_out, int32_t flags);
+
+/* Winding-number sign of voxels, batched: the interior test lagrange::volume::mesh_to_volume
+ * calls once per voxel with Sign::WindingNumber (modules/volume/src/mesh_to_volume.cpp:172-183).
+ * Voxel (i, j, k) has centre float(voxel_size * (i, j, k) + voxel_size / 2), evaluated in double
+ * as the reference's transform does (:147-149, :176-181); inside_out[t] = is_inside(centre).
+ * voxel_size is absolute (> 0): a relative size is resolved by the caller against its mesh
+ * (:132-145).  The centres are generated on the device.  omega_out is optional.
+ *
+ * fwn_sign_voxel_box: every voxel lo + (x, y, z), 0 <= x < dims[0] (fastest), y < dims[1],
+ * z < dims[2]; output t = x + dims[0] (y + dims[1] z).
+ * fwn_sign_voxels: n explicit voxels, ijk is n x 3 int32.
+ * flags: FWN_DEVICE_POINTERS applies to ijk and the outputs. */
+int fwn_sign_voxel_box(fwn_engine* e, const int32_t lo[3], const int32_t dims[3], double voxel_size,
+                       uint8_t* inside_out, float* omega_out, int32_t flags);
+int fwn_sign_voxels(fwn_engine* e, const int32_t* ijk, int64_t n, double voxel_size, uint8_t* inside_out,
+                    float* omega_out, int32_t flags);
 
 /* Single point, host memory, as the reference's per-point API.  Runs one GPU launch. */
 int fwn_solid_angle_point(fwn_engine* e, const float q[3], float* omega_out);

@@ -101,6 +101,8 @@ def load():
         "fwn_solid_angle_multi": ([VP, VP, VP, VP, VP, I32], I32),
         "fwn_solid_angle_point": ([VP, VP, VP], I32),
         "fwn_is_inside_point": ([VP, VP, VP], I32),
+        "fwn_sign_voxel_box": ([VP, VP, VP, C.c_double, VP, VP, I32], I32),
+        "fwn_sign_voxels": ([VP, VP, I64, C.c_double, VP, VP, I32], I32),
         "fwn_tree_info_get": ([VP, C.POINTER(TreeInfo)], I32),
         "fwn_export_tree": ([VP, VP, VP, VP, VP], I32),
         "fwn_query_stats": ([VP, VP, VP, I64, VP, VP, I32], I32),

@@ -74,6 +74,23 @@ def _drain(procs):
 
 CXX_DIR = os.path.join(HERE, "cpp")
 CXX_LIB = os.path.join(HERE, "liblagrange_winding.so")
+IO_LIB = os.path.join(HERE, "libfwn_io.so")
+
+
+def build_io(verbose: bool = False) -> str:
+    """The OBJ / PLY reader and writer behind include/fwn_io.h (host code, g++)."""
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall",
+           "-I", os.path.join(ROOT, "include"), "-o", IO_LIB + ".tmp",
+           os.path.join(CXX_DIR, "src", "mesh_io.cpp")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed:\n" + r.stdout)
+    os.replace(IO_LIB + ".tmp", IO_LIB)
+    return IO_LIB
+
+
 CXX_TEST = os.path.join(CXX_DIR, "build", "test_fast_winding_number")
 
 
@@ -106,3 +123,4 @@ def build_cxx(verbose: bool = False) -> str:
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
     print(build_cxx(verbose="-v" in sys.argv))
+    print(build_io(verbose="-v" in sys.argv))

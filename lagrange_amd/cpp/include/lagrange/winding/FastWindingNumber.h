@@ -50,6 +50,16 @@ public:
     /// Batched inside tests of n points; out has n entries.
     void is_inside(const std::array<float, 3>* pos, size_t n, bool* out) const;
 
+    /// Inside tests of voxel centres float(voxel_size * ijk + voxel_size / 2), the per-voxel
+    /// interior test of volume::mesh_to_volume with Sign::WindingNumber, for every voxel of the
+    /// index box lo + [0, dims) (x fastest; out has dims[0] * dims[1] * dims[2] entries).  The
+    /// centres are generated on the GPU.  voxel_size is absolute (> 0).
+    void is_inside_voxels(const std::array<int32_t, 3>& lo, const std::array<int32_t, 3>& dims, double voxel_size,
+                          bool* out) const;
+
+    /// Same for n explicit voxel indices.
+    void is_inside_voxels(const std::array<int32_t, 3>* ijk, size_t n, double voxel_size, bool* out) const;
+
 protected:
     /// Internal implementation.
     struct Impl;

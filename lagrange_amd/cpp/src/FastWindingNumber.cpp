@@ -1,8 +1,8 @@
 // lagrange::winding::FastWindingNumber over the engine's C ABI.
 //
 // Mirrors modules/winding/src/FastWindingNumber.cpp: the constructor asserts a 3D triangle mesh
-// (:202-207), copies positions to float and indices to int (:150-163) and builds the engine
-// (:168); is_inside is solid_angle / (4 pi) > 0.5 in double (:177).  Here the build and every
+// (:91-96), copies positions to float and indices to int (:39-52) and builds the engine
+// (:57); is_inside is solid_angle / (4 pi) > 0.5 in double (:66).  Here the build and every
 // query run on the GPU through fwn_create / fwn_solid_angle*.
 #include <lagrange/winding/FastWindingNumber.h>
 
@@ -98,6 +98,28 @@ void FastWindingNumber::is_inside(const std::array<float, 3>* pos, size_t n, boo
     std::vector<uint8_t> in(n);
     int st = fwn_solid_angle(eng, pos ? pos->data() : nullptr, (int64_t)n, om.data(), in.data(), 0);
     if (st != FWN_OK) raise("fwn_solid_angle", st);
+    for (size_t i = 0; i < n; ++i) out[i] = in[i] != 0;
+}
+
+void FastWindingNumber::is_inside_voxels(const std::array<int32_t, 3>& lo, const std::array<int32_t, 3>& dims,
+                                         double voxel_size, bool* out) const
+{
+    fwn_engine* eng = FWN_ENGINE_OF(m_impl);
+    size_t n = 1;
+    for (int d = 0; d < 3; ++d) n *= dims[d] > 0 ? (size_t)dims[d] : 0;
+    std::vector<uint8_t> in(n);
+    int st = fwn_sign_voxel_box(eng, lo.data(), dims.data(), voxel_size, in.data(), nullptr, 0);
+    if (st != FWN_OK) raise("fwn_sign_voxel_box", st);
+    for (size_t i = 0; i < n; ++i) out[i] = in[i] != 0;
+}
+
+void FastWindingNumber::is_inside_voxels(const std::array<int32_t, 3>* ijk, size_t n, double voxel_size,
+                                         bool* out) const
+{
+    fwn_engine* eng = FWN_ENGINE_OF(m_impl);
+    std::vector<uint8_t> in(n);
+    int st = fwn_sign_voxels(eng, ijk ? ijk->data() : nullptr, (int64_t)n, voxel_size, in.data(), nullptr, 0);
+    if (st != FWN_OK) raise("fwn_sign_voxels", st);
     for (size_t i = 0; i < n; ++i) out[i] = in[i] != 0;
 }
 

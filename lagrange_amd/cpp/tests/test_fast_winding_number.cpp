@@ -157,6 +157,53 @@ int main()
                  if (r > 1.1) CHECK(!in[i]);
              }
          }},
+        {"voxel signs: box and list equal per-point is_inside of the voxel centres", [] {
+             // mesh_to_volume's interior test (modules/volume/src/mesh_to_volume.cpp:175-183):
+             // is_inside(float(voxel_size * ijk + voxel_size / 2)) per voxel
+             FastWindingNumber e(sphere(20));
+             const double vs = 0.0731;
+             const std::array<int32_t, 3> lo{-16, -15, -17}, dims{31, 29, 33};
+             std::vector<char> box((size_t)dims[0] * dims[1] * dims[2]);
+             e.is_inside_voxels(lo, dims, vs, reinterpret_cast<bool*>(box.data()));
+             std::vector<std::array<int32_t, 3>> ijk;
+             std::vector<std::array<float, 3>> centres;
+             size_t t = 0, n_in = 0;
+             for (int z = 0; z < dims[2]; ++z)
+                 for (int y = 0; y < dims[1]; ++y)
+                     for (int x = 0; x < dims[0]; ++x, ++t) {
+                         std::array<int32_t, 3> v{lo[0] + x, lo[1] + y, lo[2] + z};
+                         std::array<float, 3> c;
+                         for (int d = 0; d < 3; ++d) {
+                             volatile double w = v[d] * vs; // keep the multiply and add separate
+                             c[d] = static_cast<float>(w + vs / 2.0);
+                         }
+                         if (t % 7 == 0) {
+                             ijk.push_back(v);
+                             centres.push_back(c);
+                         }
+                         n_in += box[t] != 0;
+                         const double r = std::sqrt(double(c[0]) * c[0] + double(c[1]) * c[1] + double(c[2]) * c[2]);
+                         if (r < 0.9) CHECK(box[t]);
+                         if (r > 1.1) CHECK(!box[t]);
+                     }
+             CHECK(n_in > 0);
+             std::vector<char> list(ijk.size()), ref(ijk.size());
+             e.is_inside_voxels(ijk.data(), ijk.size(), vs, reinterpret_cast<bool*>(list.data()));
+             e.is_inside(centres.data(), centres.size(), reinterpret_cast<bool*>(ref.data()));
+             for (size_t i = 0; i < ijk.size(); ++i) {
+                 CHECK(list[i] == ref[i]);
+                 const size_t bt = (size_t)(ijk[i][0] - lo[0]) +
+                                   (size_t)dims[0] * ((ijk[i][1] - lo[1]) + (size_t)dims[1] * (ijk[i][2] - lo[2]));
+                 CHECK(box[bt] == ref[i]);
+             }
+             bool threw = false;
+             try {
+                 e.is_inside_voxels(lo, dims, -1.0, reinterpret_cast<bool*>(box.data()));
+             } catch (const lagrange::Error&) {
+                 threw = true;
+             }
+             CHECK(threw);
+         }},
         {"concurrent const queries", [] {
              FastWindingNumber e(sphere(16));
              std::vector<float> a(64), b(64);

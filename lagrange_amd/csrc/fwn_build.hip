@@ -1,7 +1,7 @@
 // GPU tree build for the fast winding number engine (gfx950).
 //
 // Replaces UT_SolidAngle<float,float>::init, called by the reference at
-// modules/winding/src/FastWindingNumber.cpp:168 after the float/int copy of :150-163.  That
+// modules/winding/src/FastWindingNumber.cpp:57 after the float/int copy of :39-52.  That
 // dependency builds a 4-ary box-area BVH on the CPU and accumulates order-2 moments per child;
 // here the same per-node quantities are produced by a Morton-code LBVH built on the GPU:
 //

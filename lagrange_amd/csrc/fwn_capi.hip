@@ -1,7 +1,7 @@
 // C ABI of the engine (include/fwn.h).  Every entry point catches internal failures and
 // returns a status code; fwn_last_error() gives the message of the last failure on the
 // calling thread.  Argument checks mirror the reference's runtime asserts
-// (modules/winding/src/FastWindingNumber.cpp:147-148, 202-207), which throw lagrange::Error.
+// (modules/winding/src/FastWindingNumber.cpp:36-37, 91-96), which throw lagrange::Error.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -213,6 +213,31 @@ int fwn_solid_angle_multi(fwn_engine* h, const float* Q, const int64_t* q_off, f
         std::lock_guard<std::mutex> lk(e->mu);
         fwn::DeviceGuard g(e->device);
         fwn::run_queries(*e, Q, q_off, 0, omega_out, inside_out, nullptr, (flags & FWN_DEVICE_POINTERS) != 0);
+    });
+}
+
+int fwn_sign_voxel_box(fwn_engine* h, const int32_t lo[3], const int32_t dims[3], double voxel_size,
+                       uint8_t* inside_out, float* omega_out, int32_t flags)
+{
+    return guarded("fwn_sign_voxel_box", [&] {
+        fwn::Engine* e = E(h);
+        if (!lo || !dims) fwn::fail(FWN_INVALID_ARGUMENT, "NULL lo or dims");
+        if (e->n_meshes != 1) fwn::fail(FWN_INVALID_ARGUMENT, "voxel signs need a single-mesh engine");
+        std::lock_guard<std::mutex> lk(e->mu);
+        fwn::DeviceGuard g(e->device);
+        fwn::sign_voxel_box(*e, lo, dims, voxel_size, inside_out, omega_out, (flags & FWN_DEVICE_POINTERS) != 0);
+    });
+}
+
+int fwn_sign_voxels(fwn_engine* h, const int32_t* ijk, int64_t n, double voxel_size, uint8_t* inside_out,
+                    float* omega_out, int32_t flags)
+{
+    return guarded("fwn_sign_voxels", [&] {
+        fwn::Engine* e = E(h);
+        if (e->n_meshes != 1) fwn::fail(FWN_INVALID_ARGUMENT, "voxel signs need a single-mesh engine");
+        std::lock_guard<std::mutex> lk(e->mu);
+        fwn::DeviceGuard g(e->device);
+        fwn::sign_voxel_list(*e, ijk, n, voxel_size, inside_out, omega_out, (flags & FWN_DEVICE_POINTERS) != 0);
     });
 }
 

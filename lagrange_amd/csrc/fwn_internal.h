@@ -82,6 +82,7 @@ struct Engine {
     // query scratch (grown on demand)
     DBuf q_keys[2], q_vals[2], q_hist, q_rowsum, q_in, q_om, q_in8, q_off, q_boff, q_stats, q_flag;
     DBuf q_bf, q_order;
+    DBuf v_pts, v_om, v_in, v_ijk; // voxel centres generated on the device and their outputs
     int n_cu = 0;
     // deferred FP64 near-field work: (sorted point, first triangle, count) items and per-point
     // fixed-point accumulators (zero between calls)
@@ -113,6 +114,12 @@ void build_engine(Engine& e, int32_t n_meshes, const float* V_host, const int64_
 void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t nQ, float* omega, uint8_t* inside,
                  int32_t* stats, bool device_ptrs);
 void run_bruteforce(Engine& e, const float* Q, int64_t nQ, double* out, bool device_ptrs);
+
+// voxel signs (fwn_voxel.hip): centres float(voxel_size * ijk + voxel_size / 2) made on the device
+void sign_voxel_box(Engine& e, const int32_t lo[3], const int32_t dims[3], double voxel_size, uint8_t* inside,
+                    float* omega, bool device_ptrs);
+void sign_voxel_list(Engine& e, const int32_t* ijk, int64_t n, double voxel_size, uint8_t* inside, float* omega,
+                     bool device_ptrs);
 
 int grid_blocks(int64_t n, int threads);
 

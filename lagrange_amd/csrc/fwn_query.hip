@@ -1,7 +1,7 @@
 // Winding number queries on gfx950.
 //
 // Replaces UT_SolidAngle<float,float>::computeSolidAngle, called per point by the reference at
-// modules/winding/src/FastWindingNumber.cpp:177 (is_inside) and :186 (solid_angle), and the
+// modules/winding/src/FastWindingNumber.cpp:66 (is_inside) and :75 (solid_angle), and the
 // per-voxel loop of modules/volume/src/mesh_to_volume.cpp:167-183 that issues those calls.
 //
 // Points are sorted by mesh and 30-bit Morton code, then each wave64 takes 64 consecutive

@@ -609,7 +609,7 @@ void forward(Surrogate& S, hipStream_t s, const float* P, int64_t n, bool keep_f
 // gradient of the weights of layer l: gW = X^T dP (split over rows), gb = colsum(dP)
 void layer_wgrad(Surrogate& S, hipStream_t s, const uint16_t* X, int K, const uint16_t* dP, int l, int64_t n)
 {
-    const int64_t chunk = 4096;
+    const int64_t chunk = 1024; // split-K: enough workgroups to fill the chip at batch 65536
     const int nparts = (int)((n + chunk - 1) / chunk);
     S.parts.reserve(sizeof(float) * (size_t)nparts * K * kHidden);
     dim3 grid((unsigned)((K + TM - 1) / TM), (unsigned)((kHidden + TN - 1) / TN), (unsigned)nparts);
@@ -622,7 +622,7 @@ void layer_wgrad(Surrogate& S, hipStream_t s, const uint16_t* X, int K, const ui
     hipLaunchKernelGGL(k_reduce_parts, dim3(gb((int64_t)K * kHidden, 256)), dim3(256), 0, s, S.parts.as<float>(), nparts,
                        (int64_t)K * kHidden, gW);
     FWN_LAUNCH_CHECK("k_reduce_parts");
-    const int64_t rows = 512;
+    const int64_t rows = 64;
     hipLaunchKernelGGL(k_colsum, dim3(gb(n, (int)rows)), dim3(kHidden), 0, s, dP, n, kHidden, rows,
                        S.g_mlp.as<float>() + S.off_b[l]);
     FWN_LAUNCH_CHECK("k_colsum");
@@ -644,7 +644,7 @@ float forward_backward(Surrogate& S, hipStream_t s, const float* P, const float*
                        S.grid, S.R, mask_near ? S.bits.as<uint32_t>() : (const uint32_t*)nullptr,
                        S.dpred.as<float>(), S.loss.as<float>());
     FWN_LAUNCH_CHECK("k_loss_grad");
-    const int64_t rows = 512;
+    const int64_t rows = 64;
     hipLaunchKernelGGL(k_out_bwd, dim3(gb(n, (int)rows)), dim3(kHidden), 0, s, S.Xh[3].as<uint16_t>(), n, rows,
                        S.dpred.as<float>(), S.mlp.as<float>() + S.off_w[4], S.dP.as<uint16_t>(),
                        S.g_mlp.as<float>() + S.off_w[4], S.g_mlp.as<float>() + S.off_b[4]);

@@ -2,12 +2,12 @@
 
 Reference API (modules/winding/include/lagrange/winding/FastWindingNumber.h:24-108):
   * constructed from a triangle mesh; a non-3D or non-triangle mesh raises
-    (FastWindingNumber.cpp:202-207, la_runtime_assert -> lagrange::Error);
-  * positions are converted to float and indices to int (FastWindingNumber.cpp:150-163);
-  * ``solid_angle(pos)`` returns the solid angle in steradians (FastWindingNumber.cpp:180-187);
+    (FastWindingNumber.cpp:91-96, la_runtime_assert -> lagrange::Error);
+  * positions are converted to float and indices to int (FastWindingNumber.cpp:39-52);
+  * ``solid_angle(pos)`` returns the solid angle in steradians (FastWindingNumber.cpp:69-76);
   * ``is_inside(pos)`` is ``solid_angle / (4 pi) > 0.5`` evaluated in double
-    (FastWindingNumber.cpp:177);
-  * the object is move-only (copy constructor and assignment deleted, :73,:82).
+    (FastWindingNumber.cpp:66);
+  * the object is move-only (copy constructor and assignment deleted, FastWindingNumber.h:73,82).
 Additive here: batched calls over arrays (host numpy or device torch tensors) and the
 multi-mesh engine ``MultiFastWindingNumber``.  Every call runs on the GPU; there is no CPU path.
 """
@@ -40,7 +40,7 @@ def _mesh_arrays(mesh):
         raise LagrangeError("Fast winding number engine only supports 3D meshes")
     if F.ndim != 2 or F.shape[1] != 3:
         raise LagrangeError("Fast winding number engine only supports triangle meshes")
-    # the reference's internal copy: float positions, int indices (FastWindingNumber.cpp:150-163)
+    # the reference's internal copy: float positions, int indices (FastWindingNumber.cpp:39-52)
     return (np.ascontiguousarray(V, dtype=np.float32),
             np.ascontiguousarray(F.astype(np.int64, copy=False).astype(np.int32), dtype=np.int32))
 
@@ -178,6 +178,57 @@ class FastWindingNumber(_EngineBase):
 
     def is_inside_batch(self, points):
         return self.solid_angle_batch(points, with_inside=True)[1]
+
+    # ---- voxel signs (mesh_to_volume's interior test, batched) -----------------------------
+    def sign_voxel_box(self, lo, dims, voxel_size: float, with_omega: bool = False, device=None):
+        """is_inside of every voxel centre float(voxel_size * ijk + voxel_size / 2) in the index box
+        lo + [0, dims) (x fastest), as volume::mesh_to_volume's winding-number interior test
+        (modules/volume/src/mesh_to_volume.cpp:175-183).  Centres are generated on the GPU.
+        Returns an array of shape (dims[2], dims[1], dims[0]) (and omega of the same shape);
+        ``device`` = a torch device keeps the results in HBM."""
+        lo_a = np.ascontiguousarray(np.asarray(lo, dtype=np.int32).reshape(3))
+        dims_a = np.ascontiguousarray(np.asarray(dims, dtype=np.int32).reshape(3))
+        if (dims_a < 0).any():
+            raise LagrangeError("negative box dimension")
+        n = int(np.prod(dims_a.astype(np.int64)))
+        shape = (int(dims_a[2]), int(dims_a[1]), int(dims_a[0]))
+        if device is not None:
+            import torch
+            ins = torch.empty(n, dtype=torch.uint8, device=device)
+            om = torch.empty(n, dtype=torch.float32, device=device) if with_omega else None
+            torch.cuda.synchronize(device)
+            L.check(L.load().fwn_sign_voxel_box(self._h(), lo_a.ctypes.data, dims_a.ctypes.data, float(voxel_size),
+                                                _ptr(ins), _ptr(om) if om is not None else None,
+                                                L.FWN_DEVICE_POINTERS))
+            ins = ins.bool().reshape(shape)
+            return (ins, om.reshape(shape)) if with_omega else ins
+        ins = np.empty(n, np.uint8)
+        om = np.empty(n, np.float32) if with_omega else None
+        L.check(L.load().fwn_sign_voxel_box(self._h(), lo_a.ctypes.data, dims_a.ctypes.data, float(voxel_size),
+                                            ins.ctypes.data, om.ctypes.data if om is not None else None, 0))
+        ins = ins.astype(bool).reshape(shape)
+        return (ins, om.reshape(shape)) if with_omega else ins
+
+    def sign_voxels(self, ijk, voxel_size: float, with_omega: bool = False):
+        """is_inside of the centres of an (N, 3) int32 list of voxel indices (see sign_voxel_box);
+        numpy in -> numpy out, torch device tensor in -> device tensors out."""
+        if _is_torch(ijk) and ijk.is_cuda:
+            import torch
+            v = ijk.contiguous().to(torch.int32)
+            n = v.shape[0]
+            ins = torch.empty(n, dtype=torch.uint8, device=v.device)
+            om = torch.empty(n, dtype=torch.float32, device=v.device) if with_omega else None
+            torch.cuda.synchronize(v.device)
+            L.check(L.load().fwn_sign_voxels(self._h(), _ptr(v), n, float(voxel_size), _ptr(ins),
+                                             _ptr(om) if om is not None else None, L.FWN_DEVICE_POINTERS))
+            return (ins.bool(), om) if with_omega else ins.bool()
+        v = np.ascontiguousarray(np.asarray(ijk, dtype=np.int32).reshape(-1, 3))
+        n = v.shape[0]
+        ins = np.empty(n, np.uint8)
+        om = np.empty(n, np.float32) if with_omega else None
+        L.check(L.load().fwn_sign_voxels(self._h(), v.ctypes.data, n, float(voxel_size), ins.ctypes.data,
+                                         om.ctypes.data if om is not None else None, 0))
+        return (ins.astype(bool), om) if with_omega else ins.astype(bool)
 
     def query_stats(self, points):
         """(omega, stats[N,4]) with per-point work counters: tests, far terms, FP32 and FP64 triangles."""

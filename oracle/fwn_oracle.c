@@ -5,9 +5,9 @@
  *
  * What it restates, and from where:
  *  - The reference wrapper converts positions to float and indices to int before building
- *    (modules/winding/src/FastWindingNumber.cpp:150-163) and answers
+ *    (modules/winding/src/FastWindingNumber.cpp:39-52) and answers
  *    solid_angle = UT_SolidAngle<float,float>::computeSolidAngle(q) and
- *    is_inside = solid_angle / (4 pi) > 0.5 with pi a double (FastWindingNumber.cpp:171-187,
+ *    is_inside = solid_angle / (4 pi) > 0.5 with pi a double (FastWindingNumber.cpp:60-76,
  *    modules/core/include/lagrange/internal/constants.h:16).  All functions here take the
  *    same float/int32 inputs.
  *  - The engine behind it (third-party jdumas/WindingNumber @a48b8f5, absent from the

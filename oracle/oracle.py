@@ -5,7 +5,7 @@ Two independent FP64 brute forces exist: ``bruteforce`` (C, OpenMP) and
 ``bruteforce_numpy`` (vectorised numpy).  Tests require them to agree to ~1e-12 so a slip in
 either is caught.  ``Restated`` is the CPU restatement of the reference engine
 (jdumas/WindingNumber's UT_SolidAngle<float,float>, called at
-modules/winding/src/FastWindingNumber.cpp:168,177,186): 4-ary box-area BVH, order-2 far field,
+modules/winding/src/FastWindingNumber.cpp:57,66,75): 4-ary box-area BVH, order-2 far field,
 accuracy scale 2, FP32.
 """
 from __future__ import annotations

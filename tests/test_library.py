@@ -22,6 +22,20 @@ def test_library_exports_every_declared_function():
     assert set(names) <= exported
 
 
+def test_io_library_exports_every_declared_function():
+    """include/fwn_io.h is served by libfwn_io.so (host code)."""
+    import re
+    from lagrange_amd import io as mio
+    txt = open(os.path.join(os.path.dirname(L.HEADER), "fwn_io.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = sorted(set(re.findall(r"\b(fwn_[a-z0-9_]+)\s*\(", txt)))
+    assert len(names) == 7
+    mio.load_lib()
+    out = subprocess.run(["nm", "-D", "--defined-only", mio.LIB_PATH], capture_output=True, text=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    assert set(names) <= exported
+
+
 def test_default_options_match_reference_defaults():
     o = L.default_options()
     assert o.accuracy_scale == 2.0  # the reference passes no accuracy argument (default 2)
