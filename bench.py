@@ -207,6 +207,45 @@ def bench_voxel(mesh, steps: int = 5):
     }
 
 
+def bench_examples(mesh, steps: int = 5):
+    """The reference's two winding examples, batched, on the ~1M-triangle mesh: the
+    fix_orientation criterion of every facet (2 probes per facet at epsilon 1e-2, made on the GPU)
+    and sample_points_in_mesh with 16,777,216 samples in the bounding box (drawn, tested and
+    compacted on the GPU)."""
+    import numpy as np
+    import torch
+
+    from lagrange_amd.winding import FastWindingNumber
+
+    V, F = mesh.f32()
+    e = FastWindingNumber((V, F))
+    dV, dF = torch.from_numpy(V).cuda(), torch.from_numpy(F).cuda()
+    crit = e.orientation_criterion(dV, dF, 1e-2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        crit = e.orientation_criterion(dV, dF, 1e-2)
+    torch.cuda.synchronize()
+    t_or = (time.perf_counter() - t0) / steps
+    flipped = int((crit > 0.8).sum().item())
+    n = 1 << 24
+    dev = torch.device("cuda:0")
+    pts = e.sample_inside(n, 1, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pts = e.sample_inside(n, 1, device=dev)
+    torch.cuda.synchronize()
+    t_s = (time.perf_counter() - t0) / steps
+    return {
+        "fix_orientation": {"workload": f"{F.shape[0]} facets, epsilon 1e-2 (2 probes per facet)",
+                            "facets_per_s": F.shape[0] / t_or, "ms": t_or * 1e3,
+                            "facets_over_threshold": flipped},
+        "sample_points_in_mesh": {"workload": f"{n} samples uniform in the bbox", "samples_per_s": n / t_s,
+                                  "ms": t_s * 1e3, "kept_fraction": pts.shape[0] / n},
+    }
+
+
 def bench_io(mesh):
     """Host-side mesh input: the ~1M-triangle mesh written as OBJ (text) and binary PLY to a
     temporary directory, then read back by the native reader (best of 3 loads each)."""
@@ -309,9 +348,9 @@ def main():
     ap.add_argument("--leaf-max", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=4)
-    ap.add_argument("--extras", default="batched,near,voxel,io,surrogate",
+    ap.add_argument("--extras", default="batched,near,voxel,examples,io,surrogate",
                     help="comma list of extra workloads reported beside the headline: batched, near, voxel, "
-                         "io, surrogate")
+                         "examples, io, surrogate")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (scale for rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -431,6 +470,8 @@ def main():
         line["near_surface"] = bench_near_surface()
     if "voxel" in extras:
         line["voxel"] = bench_voxel(mesh)
+    if "examples" in extras:
+        line["examples"] = bench_examples(mesh)
     if "io" in extras:
         line["io"] = bench_io(mesh)
     if "surrogate" in extras:

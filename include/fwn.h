@@ -19,6 +19,8 @@
  *   fwn_sign_voxel_box /
  *   fwn_sign_voxels       <- the interior-test lambda of volume::mesh_to_volume with Sign::WindingNumber
  *                            (modules/volume/src/mesh_to_volume.cpp:175-183), voxel centres made on the GPU
+ *   fwn_orientation_criterion <- the per-facet probe pair of modules/winding/examples/fix_orientation.cpp:86-115
+ *   fwn_sample_inside     <- the rejection-sampling loop of modules/winding/examples/sample_points_in_mesh.cpp:59-76
  *   fwn_create_multi /
  *   fwn_solid_angle_multi <- no single reference call: many FastWindingNumber objects queried at once
  *   fwn_export_tree, fwn_query_stats, fwn_bruteforce_f64 -- measurement and verification hooks
@@ -162,6 +164,23 @@ int fwn_sign_voxel_box(fwn_engine* e, const int32_t lo[3], const int32_t dims[3]
                        uint8_t* inside_out, float* omega_out, int32_t flags);
 int fwn_sign_voxels(fwn_engine* e, const int32_t* ijk, int64_t n, double voxel_size, uint8_t* inside_out,
                     float* omega_out, int32_t flags);
+
+/* Orientation criterion of every facet of an input mesh against this engine's (reference) mesh,
+ * batched: modules/winding/examples/fix_orientation.cpp:86-115.  Facet f = (a, b, c) of
+ * (V: nV x 3 float, F: nF x 3 int32) gets n = normalized((b - a) x (c - a)), g = (a + b + c) / 3,
+ * and criterion_out[f] = (omega(g + eps n) - omega(g - eps n)) / (4 pi), the float difference
+ * divided in double.  The example flips facets whose criterion exceeds its threshold (0.8).
+ * The 2 nF probe points are generated on the device.  flags: FWN_DEVICE_POINTERS (V, F, out). */
+int fwn_orientation_criterion(fwn_engine* e, const float* V, int64_t nV, const int32_t* F, int64_t nF,
+                              float epsilon, double* criterion_out, int32_t flags);
+
+/* Points uniform in the box [lo, hi] kept when inside the mesh, batched:
+ * modules/winding/examples/sample_points_in_mesh.cpp:59-76.  Sample i has coordinates
+ * (hi - lo) * u + lo with u from a counter-based generator of (seed, i) (24-bit uniform in
+ * [0, 1)); the inside samples are written to points_out (capacity n_samples x 3 floats) in
+ * sample order and counted in *n_inside_out.  flags: FWN_DEVICE_POINTERS (points_out). */
+int fwn_sample_inside(fwn_engine* e, const float lo[3], const float hi[3], int64_t n_samples, uint64_t seed,
+                      float* points_out, int64_t* n_inside_out, int32_t flags);
 
 /* Single point, host memory, as the reference's per-point API.  Runs one GPU launch. */
 int fwn_solid_angle_point(fwn_engine* e, const float q[3], float* omega_out);

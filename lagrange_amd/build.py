@@ -92,6 +92,7 @@ def build_io(verbose: bool = False) -> str:
 
 
 CXX_TEST = os.path.join(CXX_DIR, "build", "test_fast_winding_number")
+CXX_EXAMPLES = ("fix_orientation", "sample_points_in_mesh")
 
 
 def build_cxx(verbose: bool = False) -> str:
@@ -111,6 +112,14 @@ def build_cxx(verbose: bool = False) -> str:
                  os.path.join(CXX_DIR, "tests", "test_fast_winding_number.cpp")] + inc +
                 ["-L", HERE, "-llagrange_winding", "-lfwn", "-Wl,-rpath," + HERE,
                  "-Wl,-rpath-link,/opt/rocm/lib"])
+    # the reference's two winding examples (need libfwn_io.so for mesh input / output)
+    if not os.path.exists(IO_LIB):
+        build_io(verbose)
+    for name in CXX_EXAMPLES:
+        cmds.append(["g++", "-std=c++17", "-O2", "-Wall", "-o", os.path.join(CXX_DIR, "build", name),
+                     os.path.join(CXX_DIR, "examples", name + ".cpp")] + inc +
+                    ["-L", HERE, "-llagrange_winding", "-lfwn", "-lfwn_io", "-Wl,-rpath," + HERE,
+                     "-Wl,-rpath-link,/opt/rocm/lib"])
     for cmd in cmds:
         if verbose:
             print(" ".join(cmd), flush=True)
@@ -122,5 +131,5 @@ def build_cxx(verbose: bool = False) -> str:
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
-    print(build_cxx(verbose="-v" in sys.argv))
     print(build_io(verbose="-v" in sys.argv))
+    print(build_cxx(verbose="-v" in sys.argv))

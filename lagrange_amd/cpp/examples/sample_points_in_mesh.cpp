@@ -1,0 +1,69 @@
+// sample_points_in_mesh: uniform samples in a mesh's bounding box kept when inside, written as
+// .xyz, as modules/winding/examples/sample_points_in_mesh.cpp does, with the samples drawn, tested
+// and compacted on the GPU in one call.
+//
+//   sample_points_in_mesh INPUT [OUTPUT=output.xyz] [-n NUM_SAMPLES=10000] [--seed SEED=0]
+#include <lagrange/winding/FastWindingNumber.h>
+
+#include <fwn_io.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+int main(int argc, char** argv)
+{
+    std::vector<std::string> pos;
+    size_t num_samples = 10000;
+    uint64_t seed = 0;
+    for (int i = 1; i < argc; ++i) {
+        std::string a = argv[i];
+        if ((a == "-n" || a == "--num-samples") && i + 1 < argc) num_samples = std::strtoull(argv[++i], nullptr, 10);
+        else if (a == "--seed" && i + 1 < argc) seed = std::strtoull(argv[++i], nullptr, 10);
+        else pos.push_back(a);
+    }
+    if (pos.empty() || pos.size() > 2) {
+        std::fprintf(stderr, "usage: %s INPUT [OUTPUT.xyz] [-n NUM_SAMPLES] [--seed SEED]\n", argv[0]);
+        return 2;
+    }
+    const std::string output = pos.size() == 2 ? pos[1] : "output.xyz";
+    if (output.size() < 4 || output.substr(output.size() - 4) != ".xyz")
+        std::fprintf(stderr, "Output file extension should be .xyz. '%s' was given.\n", output.c_str());
+    try {
+        fwn_mesh* m = nullptr;
+        if (fwn_mesh_load(pos[0].c_str(), FWN_IO_FORMAT_AUTO, 1, &m) != FWN_IO_OK)
+            throw lagrange::Error(std::string("cannot load: ") + fwn_io_last_error());
+        int64_t nv = 0, nf = 0, nc = 0;
+        fwn_mesh_sizes(m, &nv, &nf, &nc);
+        std::vector<double> V((size_t)nv * 3);
+        std::vector<int32_t> cor((size_t)nc);
+        fwn_mesh_copy(m, V.data(), nullptr, cor.data());
+        fwn_mesh_free(m);
+        lagrange::SurfaceMesh<float, uint32_t> mesh;
+        std::array<float, 3> lo{0, 0, 0}, hi{0, 0, 0};
+        for (int64_t i = 0; i < nv; ++i) {
+            const float p[3] = {(float)V[3 * i], (float)V[3 * i + 1], (float)V[3 * i + 2]};
+            mesh.add_vertex(p);
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = i ? std::min(lo[d], p[d]) : p[d];
+                hi[d] = i ? std::max(hi[d], p[d]) : p[d];
+            }
+        }
+        for (int64_t f = 0; f < nf; ++f)
+            mesh.add_triangle((uint32_t)cor[3 * f], (uint32_t)cor[3 * f + 1], (uint32_t)cor[3 * f + 2]);
+        lagrange::winding::FastWindingNumber engine(mesh);
+        std::vector<std::array<float, 3>> points(std::max<size_t>(num_samples, 1));
+        const size_t kept = engine.sample_inside(lo, hi, num_samples, seed, points.data());
+        FILE* out = std::fopen(output.c_str(), "w");
+        if (!out) throw lagrange::Error("cannot write " + output);
+        for (size_t i = 0; i < kept; ++i) std::fprintf(out, "%.9g %.9g %.9g\n", points[i][0], points[i][1], points[i][2]);
+        std::fclose(out);
+        std::printf("%zu of %zu samples inside\n", kept, num_samples);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

@@ -60,6 +60,19 @@ public:
     /// Same for n explicit voxel indices.
     void is_inside_voxels(const std::array<int32_t, 3>* ijk, size_t n, double voxel_size, bool* out) const;
 
+    /// Orientation criterion of every facet of an input triangle mesh (V: nv points, F: nf
+    /// triangles) against this mesh: (solid_angle(g + eps n) - solid_angle(g - eps n)) / (4 pi)
+    /// at the barycentre g along the unit normal n, as the fix_orientation example computes per
+    /// facet.  out has nf entries.
+    void orientation_criterion(const std::array<float, 3>* V, size_t nv, const std::array<int32_t, 3>* F, size_t nf,
+                               float epsilon, double* out) const;
+
+    /// Draws n points uniform in the box [lo, hi] (counter-based generator of seed and sample
+    /// index) and writes those inside the mesh to out (capacity n) in sample order, as the
+    /// sample_points_in_mesh example does; returns how many were written.
+    size_t sample_inside(const std::array<float, 3>& lo, const std::array<float, 3>& hi, size_t n, uint64_t seed,
+                         std::array<float, 3>* out) const;
+
 protected:
     /// Internal implementation.
     struct Impl;

@@ -123,6 +123,25 @@ void FastWindingNumber::is_inside_voxels(const std::array<int32_t, 3>* ijk, size
     for (size_t i = 0; i < n; ++i) out[i] = in[i] != 0;
 }
 
+void FastWindingNumber::orientation_criterion(const std::array<float, 3>* V, size_t nv,
+                                              const std::array<int32_t, 3>* F, size_t nf, float epsilon,
+                                              double* out) const
+{
+    int st = fwn_orientation_criterion(FWN_ENGINE_OF(m_impl), V ? V->data() : nullptr, (int64_t)nv,
+                                       F ? F->data() : nullptr, (int64_t)nf, epsilon, out, 0);
+    if (st != FWN_OK) raise("fwn_orientation_criterion", st);
+}
+
+size_t FastWindingNumber::sample_inside(const std::array<float, 3>& lo, const std::array<float, 3>& hi, size_t n,
+                                        uint64_t seed, std::array<float, 3>* out) const
+{
+    int64_t kept = 0;
+    int st = fwn_sample_inside(FWN_ENGINE_OF(m_impl), lo.data(), hi.data(), (int64_t)n, seed,
+                               out ? out->data() : nullptr, &kept, 0);
+    if (st != FWN_OK) raise("fwn_sample_inside", st);
+    return (size_t)kept;
+}
+
 // float/double x uint32/uint64, as LA_SURFACE_MESH_X (modules/core/include/lagrange/SurfaceMeshTypes.h:49-53)
 template FastWindingNumber::FastWindingNumber(const SurfaceMesh<float, uint32_t>&);
 template FastWindingNumber::FastWindingNumber(const SurfaceMesh<double, uint32_t>&);

@@ -241,6 +241,31 @@ int fwn_sign_voxels(fwn_engine* h, const int32_t* ijk, int64_t n, double voxel_s
     });
 }
 
+int fwn_orientation_criterion(fwn_engine* h, const float* V, int64_t nV, const int32_t* F, int64_t nF,
+                              float epsilon, double* criterion_out, int32_t flags)
+{
+    return guarded("fwn_orientation_criterion", [&] {
+        fwn::Engine* e = E(h);
+        if (e->n_meshes != 1) fwn::fail(FWN_INVALID_ARGUMENT, "the orientation criterion needs a single-mesh engine");
+        std::lock_guard<std::mutex> lk(e->mu);
+        fwn::DeviceGuard g(e->device);
+        fwn::orientation_criterion(*e, V, nV, F, nF, epsilon, criterion_out, (flags & FWN_DEVICE_POINTERS) != 0);
+    });
+}
+
+int fwn_sample_inside(fwn_engine* h, const float lo[3], const float hi[3], int64_t n_samples, uint64_t seed,
+                      float* points_out, int64_t* n_inside_out, int32_t flags)
+{
+    return guarded("fwn_sample_inside", [&] {
+        fwn::Engine* e = E(h);
+        if (!lo || !hi || !n_inside_out) fwn::fail(FWN_INVALID_ARGUMENT, "NULL box or count");
+        if (e->n_meshes != 1) fwn::fail(FWN_INVALID_ARGUMENT, "sampling needs a single-mesh engine");
+        std::lock_guard<std::mutex> lk(e->mu);
+        fwn::DeviceGuard g(e->device);
+        fwn::sample_inside(*e, lo, hi, n_samples, seed, points_out, n_inside_out, (flags & FWN_DEVICE_POINTERS) != 0);
+    });
+}
+
 int fwn_solid_angle_point(fwn_engine* h, const float q[3], float* omega_out)
 {
     return guarded("fwn_solid_angle_point", [&] {

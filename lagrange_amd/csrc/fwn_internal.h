@@ -82,7 +82,8 @@ struct Engine {
     // query scratch (grown on demand)
     DBuf q_keys[2], q_vals[2], q_hist, q_rowsum, q_in, q_om, q_in8, q_off, q_boff, q_stats, q_flag;
     DBuf q_bf, q_order;
-    DBuf v_pts, v_om, v_in, v_ijk; // voxel centres generated on the device and their outputs
+    DBuf v_pts, v_om, v_in, v_ijk; // query points generated on the device and their outputs
+    DBuf c_V, c_F, c_err, c_out, c_pos, c_scan, c_outp; // orientation / sampling scratch
     int n_cu = 0;
     // deferred FP64 near-field work: (sorted point, first triangle, count) items and per-point
     // fixed-point accumulators (zero between calls)
@@ -120,6 +121,12 @@ void sign_voxel_box(Engine& e, const int32_t lo[3], const int32_t dims[3], doubl
                     float* omega, bool device_ptrs);
 void sign_voxel_list(Engine& e, const int32_t* ijk, int64_t n, double voxel_size, uint8_t* inside, float* omega,
                      bool device_ptrs);
+
+// batched example callers (fwn_callers.hip)
+void orientation_criterion(Engine& e, const float* V, int64_t nV, const int32_t* F, int64_t nF, float eps,
+                           double* crit, bool device_ptrs);
+void sample_inside(Engine& e, const float lo[3], const float hi[3], int64_t n, uint64_t seed, float* out,
+                   int64_t* n_inside, bool device_ptrs);
 
 int grid_blocks(int64_t n, int threads);
 

@@ -140,6 +140,9 @@ class FastWindingNumber(_EngineBase):
         self._handle = h
         self.num_vertices = V.shape[0]
         self.num_facets = F.shape[0]
+        # float bounding box of the mesh (mesh_bbox<3> of a float mesh), for sampling
+        self.bbox = ((V.min(axis=0), V.max(axis=0)) if V.shape[0] else
+                     (np.zeros(3, np.float32), np.zeros(3, np.float32)))
 
     # ---- per-point API (as the reference) ------------------------------------------------
     def solid_angle(self, pos) -> float:
@@ -229,6 +232,47 @@ class FastWindingNumber(_EngineBase):
         L.check(L.load().fwn_sign_voxels(self._h(), v.ctypes.data, n, float(voxel_size), ins.ctypes.data,
                                          om.ctypes.data if om is not None else None, 0))
         return (ins.astype(bool), om) if with_omega else ins.astype(bool)
+
+    # ---- the reference examples, batched ---------------------------------------------------
+    def orientation_criterion(self, vertices, facets, epsilon: float = 1e-2):
+        """Per-facet criterion of modules/winding/examples/fix_orientation.cpp:86-115 for an input
+        mesh against this engine's mesh: (omega(g + eps n) - omega(g - eps n)) / (4 pi), float64,
+        probes made on the GPU.  Device tensors in -> device tensor out."""
+        if _is_torch(vertices) and vertices.is_cuda:
+            import torch
+            V = vertices.contiguous().to(torch.float32)
+            F = facets.contiguous().to(torch.int32)
+            out = torch.empty(F.shape[0], dtype=torch.float64, device=V.device)
+            torch.cuda.synchronize(V.device)
+            L.check(L.load().fwn_orientation_criterion(self._h(), _ptr(V), V.shape[0], _ptr(F), F.shape[0],
+                                                       float(epsilon), _ptr(out), L.FWN_DEVICE_POINTERS))
+            return out
+        V, F = _mesh_arrays((vertices, facets))
+        out = np.empty(F.shape[0], np.float64)
+        L.check(L.load().fwn_orientation_criterion(self._h(), V.ctypes.data, V.shape[0], F.ctypes.data, F.shape[0],
+                                                   float(epsilon), out.ctypes.data, 0))
+        return out
+
+    def sample_inside(self, num_samples: int, seed: int = 0, lo=None, hi=None, device=None):
+        """Samples uniform in the box [lo, hi] (default: the mesh's bounding box) kept when inside,
+        in sample order (modules/winding/examples/sample_points_in_mesh.cpp:59-76); sample i uses
+        the counter-based generator of (seed, i).  Returns an (M, 3) float32 array (or a device
+        tensor with ``device``)."""
+        lo = np.ascontiguousarray(np.asarray(self.bbox[0] if lo is None else lo, np.float32).reshape(3))
+        hi = np.ascontiguousarray(np.asarray(self.bbox[1] if hi is None else hi, np.float32).reshape(3))
+        n = int(num_samples)
+        cnt = C.c_int64()
+        if device is not None:
+            import torch
+            out = torch.empty((max(n, 1), 3), dtype=torch.float32, device=device)
+            torch.cuda.synchronize(device)
+            L.check(L.load().fwn_sample_inside(self._h(), lo.ctypes.data, hi.ctypes.data, n, int(seed), _ptr(out),
+                                               C.byref(cnt), L.FWN_DEVICE_POINTERS))
+            return out[:cnt.value]
+        out = np.empty((max(n, 1), 3), np.float32)
+        L.check(L.load().fwn_sample_inside(self._h(), lo.ctypes.data, hi.ctypes.data, n, int(seed), out.ctypes.data,
+                                           C.byref(cnt), 0))
+        return out[:cnt.value].copy()
 
     def query_stats(self, points):
         """(omega, stats[N,4]) with per-point work counters: tests, far terms, FP32 and FP64 triangles."""
