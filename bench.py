@@ -53,6 +53,20 @@ def _read_traffic():
     return None
 
 
+def _host_cpu():
+    model, sockets = "unknown", set()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name") and model == "unknown":
+                model = line.split(":", 1)[1].strip()
+            if line.startswith("physical id"):
+                sockets.add(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return {"model": model, "sockets": max(len(sockets), 1), "nproc": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0))}
+
+
 def cpu_baseline(V, F, q, sample_stride: int, threads: int):
     from oracle import oracle as O
     sub = q[::sample_stride].copy()
@@ -61,19 +75,30 @@ def cpu_baseline(V, F, q, sample_stride: int, threads: int):
     build_s = time.perf_counter() - t0
     R.solid_angle(sub[:4096], nthreads=threads)  # warm-up
     times = []
-    for _ in range(3):
+    for _ in range(5):
         t0 = time.perf_counter()
         R.solid_angle(sub, nthreads=threads)
         times.append(time.perf_counter() - t0)
-    dt = sorted(times)[1]
+    dt = sorted(times)[2]
+    one = sub[::16].copy()
+    t1 = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        R.solid_angle(one, nthreads=1)
+        t1.append(time.perf_counter() - t0)
+    dt1 = sorted(t1)[1]
     return {
         "value": sub.shape[0] / dt,
         "unit": "queries/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"every {sample_stride}th grid point ({sub.shape[0]} points), median of 3 runs; "
+        "sample": (f"every {sample_stride}th grid point" if sample_stride > 1 else "all grid points") +
+                  f" ({sub.shape[0]} points), median of 5 runs; "
                   f"CPU restatement of the reference engine (4-ary box-area BVH, order 2, beta 2, FP32), "
                   f"OpenMP over points; build {build_s:.2f} s single-threaded, not included",
+        "single_thread_value": one.shape[0] / dt1,
+        "single_thread_sample": f"every {16 * sample_stride}th grid point ({one.shape[0]} points), median of 3",
+        "host": _host_cpu(),
     }
 
 
@@ -246,6 +271,38 @@ def bench_examples(mesh, steps: int = 5):
     }
 
 
+def bench_closest(mesh, q, steps: int = 5):
+    """Closest point / unsigned distance (bvh::TriangleAABBTree::get_closest_point, batched) for the
+    headline grid points on the ~1M-triangle mesh, FP64 point-triangle distances, results in HBM;
+    checked against the brute-force oracle on a small strided sample."""
+    import numpy as np
+    import torch
+
+    from lagrange_amd.winding import FastWindingNumber
+
+    V, F = mesh.f32()
+    e = FastWindingNumber((V, F))
+    qd = torch.from_numpy(q).cuda()
+    d2, fi, cp = e.closest_point(qd)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        d2, fi, cp = e.closest_point(qd)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    out = {"workload": f"{F.shape[0]} triangles, {q.shape[0]} grid points (bbox x1.2)",
+           "points_per_s": q.shape[0] / dt, "ms": dt * 1e3}
+    try:
+        from oracle import oracle as O
+        idx = np.arange(0, q.shape[0], q.shape[0] // 64)
+        od2, _, _ = O.closest(V, F, q[idx])
+        out["max_rel_err_vs_bruteforce"] = float(np.max(np.abs(d2.cpu().numpy()[idx] - od2) / np.maximum(od2, 1e-300)))
+        out["sample"] = int(idx.size)
+    except Exception as ex:  # the checker is optional in a bench run
+        out["check"] = f"skipped: {ex}"
+    return out
+
+
 def bench_io(mesh):
     """Host-side mesh input: the ~1M-triangle mesh written as OBJ (text) and binary PLY to a
     temporary directory, then read back by the native reader (best of 3 loads each)."""
@@ -347,10 +404,10 @@ def main():
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--leaf-max", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-stride", type=int, default=4)
-    ap.add_argument("--extras", default="batched,near,voxel,examples,io,surrogate",
+    ap.add_argument("--cpu-stride", type=int, default=1)
+    ap.add_argument("--extras", default="batched,near,voxel,examples,closest,io,surrogate",
                     help="comma list of extra workloads reported beside the headline: batched, near, voxel, "
-                         "examples, io, surrogate")
+                         "examples, closest, io, surrogate")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (scale for rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -472,6 +529,8 @@ def main():
         line["voxel"] = bench_voxel(mesh)
     if "examples" in extras:
         line["examples"] = bench_examples(mesh)
+    if "closest" in extras:
+        line["closest"] = bench_closest(mesh, q)
     if "io" in extras:
         line["io"] = bench_io(mesh)
     if "surrogate" in extras:

@@ -21,6 +21,7 @@
  *                            (modules/volume/src/mesh_to_volume.cpp:175-183), voxel centres made on the GPU
  *   fwn_orientation_criterion <- the per-facet probe pair of modules/winding/examples/fix_orientation.cpp:86-115
  *   fwn_sample_inside     <- the rejection-sampling loop of modules/winding/examples/sample_points_in_mesh.cpp:59-76
+ *   fwn_closest_point     <- bvh::TriangleAABBTree::get_closest_point (modules/bvh/src/TriangleAABBTree.cpp:86-127)
  *   fwn_create_multi /
  *   fwn_solid_angle_multi <- no single reference call: many FastWindingNumber objects queried at once
  *   fwn_export_tree, fwn_query_stats, fwn_bruteforce_f64 -- measurement and verification hooks
@@ -181,6 +182,17 @@ int fwn_orientation_criterion(fwn_engine* e, const float* V, int64_t nV, const i
  * sample order and counted in *n_inside_out.  flags: FWN_DEVICE_POINTERS (points_out). */
 int fwn_sample_inside(fwn_engine* e, const float lo[3], const float hi[3], int64_t n_samples, uint64_t seed,
                       float* points_out, int64_t* n_inside_out, int32_t flags);
+
+/* Closest point on the mesh (unsigned distance) of nQ points, over the same tree:
+ * lagrange::bvh::TriangleAABBTree::get_closest_point (modules/bvh/src/TriangleAABBTree.cpp:86-127)
+ * with the per-triangle squared distance of lagrange::point_triangle_squared_distance (Eberly's
+ * region method in double, modules/core/include/lagrange/utils/point_triangle_squared_distance.h).
+ * sq_dist_out (double) and facet_out (input facet index) per point; closest_out (nQ x 3 double)
+ * is optional.  The reference rounds both to the mesh scalar; here they stay double.  On an
+ * empty mesh facet_out is -1 and sq_dist_out +inf.  Equidistant facets may be reported by either
+ * index.  flags: FWN_DEVICE_POINTERS. */
+int fwn_closest_point(fwn_engine* e, const float* Q, int64_t nQ, double* sq_dist_out, int32_t* facet_out,
+                      double* closest_out, int32_t flags);
 
 /* Single point, host memory, as the reference's per-point API.  Runs one GPU launch. */
 int fwn_solid_angle_point(fwn_engine* e, const float q[3], float* omega_out);

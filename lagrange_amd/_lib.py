@@ -105,6 +105,7 @@ def load():
         "fwn_sign_voxels": ([VP, VP, I64, C.c_double, VP, VP, I32], I32),
         "fwn_orientation_criterion": ([VP, VP, I64, VP, I64, C.c_float, VP, I32], I32),
         "fwn_sample_inside": ([VP, VP, VP, I64, C.c_uint64, VP, VP, I32], I32),
+        "fwn_closest_point": ([VP, VP, I64, VP, VP, VP, I32], I32),
         "fwn_tree_info_get": ([VP, C.POINTER(TreeInfo)], I32),
         "fwn_export_tree": ([VP, VP, VP, VP, VP], I32),
         "fwn_query_stats": ([VP, VP, VP, I64, VP, VP, I32], I32),

@@ -73,6 +73,11 @@ public:
     size_t sample_inside(const std::array<float, 3>& lo, const std::array<float, 3>& hi, size_t n, uint64_t seed,
                          std::array<float, 3>* out) const;
 
+    /// Closest point on the mesh of n points (bvh::TriangleAABBTree::get_closest_point, batched):
+    /// squared distance, facet index and (when closest is not null) the closest point.
+    void closest_point(const std::array<float, 3>* pos, size_t n, double* sq_dist, int32_t* facet,
+                       std::array<double, 3>* closest) const;
+
 protected:
     /// Internal implementation.
     struct Impl;

@@ -142,6 +142,14 @@ size_t FastWindingNumber::sample_inside(const std::array<float, 3>& lo, const st
     return (size_t)kept;
 }
 
+void FastWindingNumber::closest_point(const std::array<float, 3>* pos, size_t n, double* sq_dist, int32_t* facet,
+                                      std::array<double, 3>* closest) const
+{
+    int st = fwn_closest_point(FWN_ENGINE_OF(m_impl), pos ? pos->data() : nullptr, (int64_t)n, sq_dist, facet,
+                               closest ? closest->data() : nullptr, 0);
+    if (st != FWN_OK) raise("fwn_closest_point", st);
+}
+
 // float/double x uint32/uint64, as LA_SURFACE_MESH_X (modules/core/include/lagrange/SurfaceMeshTypes.h:49-53)
 template FastWindingNumber::FastWindingNumber(const SurfaceMesh<float, uint32_t>&);
 template FastWindingNumber::FastWindingNumber(const SurfaceMesh<double, uint32_t>&);

@@ -204,6 +204,21 @@ int main()
              }
              CHECK(threw);
          }},
+        {"closest point on the cube", [] {
+             FastWindingNumber e(cube<float, uint32_t>());
+             const std::array<float, 3> q[4] = {{0.5f, 0.5f, 2.0f}, {-1.0f, 0.5f, 0.5f}, {2.0f, 2.0f, 2.0f},
+                                                {0.5f, 0.5f, 0.75f}};
+             const double want[4] = {1.0, 1.0, 3.0, 0.0625};
+             double d2[4];
+             int32_t f[4];
+             std::array<double, 3> cp[4];
+             e.closest_point(q, 4, d2, f, cp);
+             for (int i = 0; i < 4; ++i) {
+                 CHECK(std::fabs(d2[i] - want[i]) < 1e-12);
+                 CHECK(f[i] >= 0 && f[i] < 12);
+             }
+             CHECK(std::fabs(cp[0][2] - 1.0) < 1e-12 && std::fabs(cp[2][0] - 1.0) < 1e-12);
+         }},
         {"concurrent const queries", [] {
              FastWindingNumber e(sphere(16));
              std::vector<float> a(64), b(64);

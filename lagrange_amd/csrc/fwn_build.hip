@@ -16,7 +16,8 @@
 //                   small nodes and FP64 for nodes with >= fp64_moment_min triangles
 //   k_collapse_*    level by level, each wide node takes the binary node's children and
 //                   repeatedly opens the largest non-leaf slot until it has tree_width slots
-//   k_write_slots   FP64 records -> the 32-word FP32 slot format of fwn.h
+//   k_write_slots   FP64 records -> the 32-word FP32 slot format of fwn.h, plus each slot's box
+//                   (6 floats, rounded outward) for distance queries
 //
 // Several meshes are built together: their keys carry the mesh id above the Morton bits, so
 // each mesh is one subtree of a single radix tree and every kernel runs once for all meshes.
@@ -461,20 +462,27 @@ __global__ __launch_bounds__(BT) void k_write_slots(int64_t W, int width, const 
                                                     const double* __restrict__ rec,
                                                     const int32_t* __restrict__ first,
                                                     const int32_t* __restrict__ last, int64_t n,
-                                                    uint32_t* __restrict__ slots)
+                                                    uint32_t* __restrict__ slots, float* __restrict__ boxes)
 {
     int64_t idx = (int64_t)blockIdx.x * BT + threadIdx.x;
     if (idx >= W * width) return;
     int64_t g = idx / width;
     int s = (int)(idx % width);
     uint32_t* w = slots + idx * kSlotWords;
+    float* bx = boxes + idx * 6;
     int64_t src = g * kWidth + s; // collapse scratch keeps kWidth entries per node
     int32_t u = wide_slot_bin[src];
     if (s >= wide_ns[g] || u < 0) {
         for (int k = 0; k < kSlotWords; ++k) w[k] = 0u;
+        for (int k = 0; k < 6; ++k) bx[k] = 0.0f;
         return;
     }
     const double* r = rec + (int64_t)u * kRecWords;
+    // the slot's box for distance queries, rounded outward so it still contains the triangles
+    for (int d = 0; d < 3; ++d) {
+        bx[d] = __double2float_rd(r[R_LO + d]);
+        bx[3 + d] = __double2float_ru(r[R_HI + d]);
+    }
     float f[27];
     double r2 = 0.0;
     for (int d = 0; d < 3; ++d) {
@@ -593,6 +601,7 @@ void build_engine(Engine& e, int32_t M, const float* V_host, const int64_t* v_of
 
     if (nF == 0) {
         e.slots.alloc(sizeof(uint32_t) * width * kSlotWords);
+        e.boxes.alloc(sizeof(float) * width * 6);
         e.n_nodes = 0;
         e.max_depth = 0;
         FWN_HIP(hipMemcpyAsync(e.roots.p, e.h_roots.data(), sizeof(int32_t) * 2 * M, hipMemcpyHostToDevice, s));
@@ -708,9 +717,10 @@ void build_engine(Engine& e, int32_t M, const float* V_host, const int64_t* v_of
     e.n_nodes = W;
     e.max_depth = depth;
     e.slots.alloc(sizeof(uint32_t) * width * kSlotWords * (size_t)std::max<int64_t>(W, 1));
+    e.boxes.alloc(sizeof(float) * width * 6 * (size_t)std::max<int64_t>(W, 1));
     hipLaunchKernelGGL(k_write_slots, dim3(grid_blocks(W * width, BT)), dim3(BT), 0, s, W, width, wsb.as<int32_t>(),
                        wns.as<int32_t>(), cw.as<int32_t>(), rec.as<double>(), first.as<int32_t>(), last.as<int32_t>(),
-                       n, e.slots.as<uint32_t>());
+                       n, e.slots.as<uint32_t>(), e.boxes.as<float>());
     FWN_LAUNCH_CHECK("k_write_slots");
     std::vector<int32_t> ns_root(level0.size());
     if (!level0.empty())

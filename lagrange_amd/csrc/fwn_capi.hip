@@ -266,6 +266,17 @@ int fwn_sample_inside(fwn_engine* h, const float lo[3], const float hi[3], int64
     });
 }
 
+int fwn_closest_point(fwn_engine* h, const float* Q, int64_t nQ, double* sq_dist_out, int32_t* facet_out,
+                      double* closest_out, int32_t flags)
+{
+    return guarded("fwn_closest_point", [&] {
+        fwn::Engine* e = E(h);
+        std::lock_guard<std::mutex> lk(e->mu);
+        fwn::DeviceGuard g(e->device);
+        fwn::run_closest(*e, Q, nQ, sq_dist_out, facet_out, closest_out, (flags & FWN_DEVICE_POINTERS) != 0);
+    });
+}
+
 int fwn_solid_angle_point(fwn_engine* h, const float q[3], float* omega_out)
 {
     return guarded("fwn_solid_angle_point", [&] {

@@ -72,6 +72,7 @@ struct Engine {
 
     // tree (resident)
     DBuf slots;      // n_nodes * width * kSlotWords u32
+    DBuf boxes;      // n_nodes * width * 6 floats: each slot's box (lo, hi), for distance queries
     DBuf tris;       // n_tris * kTriWords u32
     DBuf roots;      // 2 int32 per mesh: wide root, occupied slots
     DBuf mesh_info;  // per mesh: 8 floats: qbox lo[3], qbox inv[3], tau, pad
@@ -84,6 +85,7 @@ struct Engine {
     DBuf q_bf, q_order;
     DBuf v_pts, v_om, v_in, v_ijk; // query points generated on the device and their outputs
     DBuf c_V, c_F, c_err, c_out, c_pos, c_scan, c_outp; // orientation / sampling scratch
+    DBuf d_d2, d_tri, d_cp;                               // closest-point outputs for host callers
     int n_cu = 0;
     // deferred FP64 near-field work: (sorted point, first triangle, count) items and per-point
     // fixed-point accumulators (zero between calls)
@@ -115,6 +117,13 @@ void build_engine(Engine& e, int32_t n_meshes, const float* V_host, const int64_
 void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t nQ, float* omega, uint8_t* inside,
                  int32_t* stats, bool device_ptrs);
 void run_bruteforce(Engine& e, const float* Q, int64_t nQ, double* out, bool device_ptrs);
+// Morton order of nQ device points of a single-mesh engine: returns the permutation (sorted
+// position -> original index) in the engine's query scratch, valid until the next query call.
+const uint32_t* sort_points_single(Engine& e, const float* dQ, int64_t nQ);
+
+// closest point on the mesh of a single-mesh engine (fwn_closest.hip): squared distance (double),
+// original facet index (-1 for an empty mesh) and, when cp is not NULL, the point (3 doubles)
+void run_closest(Engine& e, const float* Q, int64_t nQ, double* d2, int32_t* tri, double* cp, bool device_ptrs);
 
 // voxel signs (fwn_voxel.hip): centres float(voxel_size * ijk + voxel_size / 2) made on the device
 void sign_voxel_box(Engine& e, const int32_t lo[3], const int32_t dims[3], double voxel_size, uint8_t* inside,
@@ -211,6 +220,46 @@ __host__ __device__ inline float ord2f(int32_t i)
     float f;
     __builtin_memcpy(&f, &j, 4);
     return f;
+}
+
+// Wave-private traversal stack of (node entry, 64-bit lane mask), one entry per lane and bank:
+// entry k lives in lane k % 64 of bank k / 64 (3 banks = kStack entries), so a push or pop is a
+// select / v_readlane on uniform indices and the stack never touches scratch memory.
+struct WaveStack {
+    uint32_t n0 = 0, n1 = 0, n2 = 0, l0 = 0, l1 = 0, l2 = 0, h0 = 0, h1 = 0, h2 = 0;
+};
+
+// Branch-free: every bank is updated / read under uniform predicates, so nothing invites the
+// optimiser to turn the banks into an indexed private array.
+__device__ __forceinline__ void stack_push(WaveStack& S, int sp, int lane, uint32_t e, uint64_t mk)
+{
+    const bool me = lane == (sp & 63);
+    const bool b0 = me && sp < 64, b1 = me && sp >= 64 && sp < 128, b2 = me && sp >= 128;
+    const uint32_t mlo = (uint32_t)mk, mhi = (uint32_t)(mk >> 32);
+    S.n0 = b0 ? e : S.n0;
+    S.l0 = b0 ? mlo : S.l0;
+    S.h0 = b0 ? mhi : S.h0;
+    S.n1 = b1 ? e : S.n1;
+    S.l1 = b1 ? mlo : S.l1;
+    S.h1 = b1 ? mhi : S.h1;
+    S.n2 = b2 ? e : S.n2;
+    S.l2 = b2 ? mlo : S.l2;
+    S.h2 = b2 ? mhi : S.h2;
+}
+
+__device__ __forceinline__ void stack_pop(const WaveStack& S, int sp, uint32_t& e, uint64_t& mk)
+{
+    const int l = sp & 63;
+    const uint32_t a0 = __builtin_amdgcn_readlane(S.n0, l), a1 = __builtin_amdgcn_readlane(S.n1, l),
+                   a2 = __builtin_amdgcn_readlane(S.n2, l);
+    const uint32_t c0 = __builtin_amdgcn_readlane(S.l0, l), c1 = __builtin_amdgcn_readlane(S.l1, l),
+                   c2 = __builtin_amdgcn_readlane(S.l2, l);
+    const uint32_t d0 = __builtin_amdgcn_readlane(S.h0, l), d1 = __builtin_amdgcn_readlane(S.h1, l),
+                   d2 = __builtin_amdgcn_readlane(S.h2, l);
+    const uint32_t lo = sp < 64 ? c0 : (sp < 128 ? c1 : c2);
+    const uint32_t hi = sp < 64 ? d0 : (sp < 128 ? d1 : d2);
+    e = sp < 64 ? a0 : (sp < 128 ? a1 : a2);
+    mk = ((uint64_t)hi << 32) | lo;
 }
 
 // XCD-aware block remap: hardware hands consecutive workgroups to the 8 XCDs round-robin;

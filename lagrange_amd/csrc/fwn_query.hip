@@ -168,43 +168,6 @@ __device__ __forceinline__ float far_field_r(const SlotRegs& r, float dx, float 
 
 // ---- wave stack in VGPRs: entry i sits in lane (i & 63) of bank (i >> 6) -----------------
 // Named registers (no arrays) so the banks stay in VGPRs rather than private memory.
-struct WaveStack {
-    uint32_t n0 = 0, n1 = 0, n2 = 0, l0 = 0, l1 = 0, l2 = 0, h0 = 0, h1 = 0, h2 = 0;
-};
-
-// Branch-free: every bank is updated / read under uniform predicates, so nothing invites the
-// optimiser to turn the banks into an indexed private array.
-__device__ __forceinline__ void stack_push(WaveStack& S, int sp, int lane, uint32_t e, uint64_t mk)
-{
-    const bool me = lane == (sp & 63);
-    const bool b0 = me && sp < 64, b1 = me && sp >= 64 && sp < 128, b2 = me && sp >= 128;
-    const uint32_t mlo = (uint32_t)mk, mhi = (uint32_t)(mk >> 32);
-    S.n0 = b0 ? e : S.n0;
-    S.l0 = b0 ? mlo : S.l0;
-    S.h0 = b0 ? mhi : S.h0;
-    S.n1 = b1 ? e : S.n1;
-    S.l1 = b1 ? mlo : S.l1;
-    S.h1 = b1 ? mhi : S.h1;
-    S.n2 = b2 ? e : S.n2;
-    S.l2 = b2 ? mlo : S.l2;
-    S.h2 = b2 ? mhi : S.h2;
-}
-
-__device__ __forceinline__ void stack_pop(const WaveStack& S, int sp, uint32_t& e, uint64_t& mk)
-{
-    const int l = sp & 63;
-    const uint32_t a0 = __builtin_amdgcn_readlane(S.n0, l), a1 = __builtin_amdgcn_readlane(S.n1, l),
-                   a2 = __builtin_amdgcn_readlane(S.n2, l);
-    const uint32_t c0 = __builtin_amdgcn_readlane(S.l0, l), c1 = __builtin_amdgcn_readlane(S.l1, l),
-                   c2 = __builtin_amdgcn_readlane(S.l2, l);
-    const uint32_t d0 = __builtin_amdgcn_readlane(S.h0, l), d1 = __builtin_amdgcn_readlane(S.h1, l),
-                   d2 = __builtin_amdgcn_readlane(S.h2, l);
-    const uint32_t lo = sp < 64 ? c0 : (sp < 128 ? c1 : c2);
-    const uint32_t hi = sp < 64 ? d0 : (sp < 128 ? d1 : d2);
-    e = sp < 64 ? a0 : (sp < 128 ? a1 : a2);
-    mk = ((uint64_t)hi << 32) | lo;
-}
-
 // One batch of up to 64 points (one per lane) of one mesh, walked through that mesh's tree.
 template <bool STATS>
 __device__ __forceinline__ void traverse_batch(
@@ -616,6 +579,26 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
     FWN_HIP(hipEventElapsedTime(&t_all, e.ev[0], e.ev[3]));
     e.last_traverse_ms = t_tr;
     e.last_total_ms = t_all;
+}
+
+const uint32_t* sort_points_single(Engine& e, const float* dQ, int64_t nQ)
+{
+    if (e.n_meshes != 1) fail(FWN_INVALID_ARGUMENT, "a single-mesh engine is required");
+    if (nQ > (int64_t)UINT32_MAX / 2) fail(FWN_LIMIT_EXCEEDED, "more than 2^31 points in one call");
+    hipStream_t s = e.stream;
+    for (int k = 0; k < 2; ++k) {
+        e.q_keys[k].reserve(sizeof(uint32_t) * nQ);
+        e.q_vals[k].reserve(sizeof(uint32_t) * nQ);
+    }
+    uint32_t* kp = e.q_keys[0].as<uint32_t>();
+    uint32_t* vp = e.q_vals[0].as<uint32_t>();
+    uint32_t* kp2 = e.q_keys[1].as<uint32_t>();
+    uint32_t* vp2 = e.q_vals[1].as<uint32_t>();
+    hipLaunchKernelGGL(k_query_keys, dim3(grid_blocks(nQ, QT)), dim3(QT), 0, s, dQ, nQ, (const int64_t*)nullptr, 1, 0,
+                       e.mesh_info.as<float>(), kp, vp);
+    FWN_LAUNCH_CHECK("k_query_keys");
+    radix_sort_pairs32(kp, vp, kp2, vp2, nQ, 8, 32, e.q_hist, e.q_rowsum, s);
+    return vp;
 }
 
 void run_bruteforce(Engine& e, const float* Q, int64_t nQ, double* out, bool dev)

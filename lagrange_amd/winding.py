@@ -274,6 +274,31 @@ class FastWindingNumber(_EngineBase):
                                            C.byref(cnt), 0))
         return out[:cnt.value].copy()
 
+    # ---- closest point (bvh::TriangleAABBTree::get_closest_point, batched) ----------------
+    def closest_point(self, points, with_point: bool = True):
+        """(squared distance float64, facet int32, closest point (N, 3) float64) per point, over
+        the engine's tree (modules/bvh/src/TriangleAABBTree.cpp:86-127); an empty mesh gives
+        facet -1 and +inf.  numpy in -> numpy out, torch device tensor in -> device tensors out."""
+        if _is_torch(points) and points.is_cuda:
+            import torch
+            q = points.contiguous().to(torch.float32)
+            n = q.shape[0]
+            d2 = torch.empty(n, dtype=torch.float64, device=q.device)
+            fi = torch.empty(n, dtype=torch.int32, device=q.device)
+            cp = torch.empty((n, 3), dtype=torch.float64, device=q.device) if with_point else None
+            torch.cuda.synchronize(q.device)
+            L.check(L.load().fwn_closest_point(self._h(), _ptr(q), n, _ptr(d2), _ptr(fi),
+                                               _ptr(cp) if cp is not None else None, L.FWN_DEVICE_POINTERS))
+            return (d2, fi, cp) if with_point else (d2, fi)
+        q = np.ascontiguousarray(np.asarray(points, dtype=np.float32).reshape(-1, 3))
+        n = q.shape[0]
+        d2 = np.empty(n, np.float64)
+        fi = np.empty(n, np.int32)
+        cp = np.empty((n, 3), np.float64) if with_point else None
+        L.check(L.load().fwn_closest_point(self._h(), q.ctypes.data, n, d2.ctypes.data, fi.ctypes.data,
+                                           cp.ctypes.data if cp is not None else None, 0))
+        return (d2, fi, cp) if with_point else (d2, fi)
+
     def query_stats(self, points):
         """(omega, stats[N,4]) with per-point work counters: tests, far terms, FP32 and FP64 triangles."""
         q = np.ascontiguousarray(np.asarray(points, dtype=np.float32).reshape(-1, 3))

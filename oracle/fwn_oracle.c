@@ -29,6 +29,11 @@
  *                        by the GPU engine), optional FP64 near-field, per-point work counters
  *  lbvh_sorted_order     30-bit Morton codes of triangle centroids + stable sort: the triangle
  *                        order a Morton LBVH must produce (bit-exact check of the GPU sort)
+ *  oracle_closest        brute-force closest point on a mesh: minimum over all facets of the
+ *                        point-triangle squared distance of
+ *                        modules/core/include/lagrange/utils/point_triangle_squared_distance.h
+ *                        (Eberly's region method in double; degenerate triangles by their edges,
+ *                        modules/core/include/lagrange/utils/point_segment_squared_distance.h)
  */
 #include <math.h>
 #include <stdint.h>
@@ -777,4 +782,169 @@ API void lbvh_sorted_order(int32_t n_meshes, const float* V, const int64_t* v_of
     }
     free(a);
     free(cen);
+}
+
+
+/* ------------------------------------------------------------------------------------ */
+/* closest point (brute force)                                                           */
+/* ------------------------------------------------------------------------------------ */
+
+/* squared distance from q to segment (a, b), parameter u of the closest point a + u (b - a) */
+static double seg_d2(const double* q, const double* a, const double* b, double* u_out)
+{
+    double ab[3], aq[3];
+    for (int d = 0; d < 3; ++d) {
+        ab[d] = b[d] - a[d];
+        aq[d] = q[d] - a[d];
+    }
+    const double l2 = ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2];
+    const double t = aq[0] * ab[0] + aq[1] * ab[1] + aq[2] * ab[2];
+    double u = 0.0;
+    if (!(t <= 0.0 || l2 == 0.0)) u = t > l2 ? 1.0 : t / l2;
+    double r = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        const double x = q[d] - (a[d] + u * ab[d]);
+        r += x * x;
+    }
+    *u_out = u;
+    return r;
+}
+
+/* Eberly's minimisation of |v0 + s e0 + t e1 - q|^2 over the triangle, by the region of the
+ * unconstrained minimiser (s, t) (point_triangle_squared_distance.h:31-280). */
+static double tri_closest(const double* v0, const double* v1, const double* v2, const double* q, double* so,
+                          double* to)
+{
+    double df[3], e0[3], e1[3];
+    for (int d = 0; d < 3; ++d) {
+        df[d] = v0[d] - q[d];
+        e0[d] = v1[d] - v0[d];
+        e1[d] = v2[d] - v0[d];
+    }
+    const double a00 = e0[0] * e0[0] + e0[1] * e0[1] + e0[2] * e0[2];
+    const double a01 = e0[0] * e1[0] + e0[1] * e1[1] + e0[2] * e1[2];
+    const double a11 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
+    const double b0 = df[0] * e0[0] + df[1] * e0[1] + df[2] * e0[2];
+    const double b1 = df[0] * e1[0] + df[1] * e1[1] + df[2] * e1[2];
+    const double c = df[0] * df[0] + df[1] * df[1] + df[2] * df[2];
+    const double det = fabs(a00 * a11 - a01 * a01);
+    double s = a01 * b1 - a11 * b0, t = a01 * b0 - a00 * b1, r;
+    if (det < 1e-30) {
+        const double* E[3][2] = {{v0, v1}, {v0, v2}, {v1, v2}};
+        r = INFINITY;
+        for (int k = 0; k < 3; ++k) {
+            double u;
+            const double dd = seg_d2(q, E[k][0], E[k][1], &u);
+            if (dd < r) {
+                r = dd;
+                *so = k == 0 ? u : (k == 1 ? 0.0 : 1.0 - u);
+                *to = k == 0 ? 0.0 : u;
+            }
+        }
+        return r;
+    }
+    /* on an edge: minimise the 1D quadratic a s^2 + 2 b s + c with s clamped to [0, 1] */
+#define EDGE(ss, aa, bb, rr)                                                                   \
+    do {                                                                                       \
+        if ((bb) >= 0.0) { ss = 0.0; rr = c; }                                                 \
+        else if (-(bb) >= (aa)) { ss = 1.0; rr = (aa) + 2.0 * (bb) + c; }                      \
+        else { ss = -(bb) / (aa); rr = (bb) * ss + c; }                                        \
+    } while (0)
+#define FULLQ(ss, tt) ((ss) * (a00 * (ss) + a01 * (tt) + 2.0 * b0) + (tt) * (a01 * (ss) + a11 * (tt) + 2.0 * b1) + c)
+    if (s + t <= det) {
+        if (s < 0.0) {
+            if (t < 0.0) {                   /* region 4: vertex v0 neighbourhood */
+                if (b0 < 0.0) { t = 0.0; EDGE(s, a00, b0, r); }
+                else { s = 0.0; EDGE(t, a11, b1, r); }
+            } else { s = 0.0; EDGE(t, a11, b1, r); }          /* region 3 */
+        } else if (t < 0.0) { t = 0.0; EDGE(s, a00, b0, r); } /* region 5 */
+        else {                                                 /* region 0 */
+            const double inv = 1.0 / det;
+            s = s * inv;
+            t = t * inv;
+            r = FULLQ(s, t);
+        }
+    } else {
+        const double denom = a00 - 2.0 * a01 + a11;
+        if (s < 0.0) {                                          /* region 2 */
+            const double tmp0 = a01 + b0, tmp1 = a11 + b1;
+            if (tmp1 > tmp0) {
+                const double numer = tmp1 - tmp0;
+                if (numer >= denom) { s = 1.0; t = 0.0; r = a00 + 2.0 * b0 + c; }
+                else { s = numer / denom; t = 1.0 - s; r = FULLQ(s, t); }
+            } else {
+                s = 0.0;
+                if (tmp1 <= 0.0) { t = 1.0; r = a11 + 2.0 * b1 + c; }
+                else if (b1 >= 0.0) { t = 0.0; r = c; }
+                else { t = -b1 / a11; r = b1 * t + c; }
+            }
+        } else if (t < 0.0) {                                   /* region 6 */
+            const double tmp0 = a01 + b1, tmp1 = a00 + b0;
+            if (tmp1 > tmp0) {
+                const double numer = tmp1 - tmp0;
+                if (numer >= denom) { t = 1.0; s = 0.0; r = a11 + 2.0 * b1 + c; }
+                else { t = numer / denom; s = 1.0 - t; r = FULLQ(s, t); }
+            } else {
+                t = 0.0;
+                if (tmp1 <= 0.0) { s = 1.0; r = a00 + 2.0 * b0 + c; }
+                else if (b0 >= 0.0) { s = 0.0; r = c; }
+                else { s = -b0 / a00; r = b0 * s + c; }
+            }
+        } else {                                                /* region 1 */
+            const double numer = a11 + b1 - a01 - b0;
+            if (numer <= 0.0) { s = 0.0; t = 1.0; r = a11 + 2.0 * b1 + c; }
+            else if (numer >= denom) { s = 1.0; t = 0.0; r = a00 + 2.0 * b0 + c; }
+            else { s = numer / denom; t = 1.0 - s; r = FULLQ(s, t); }
+        }
+    }
+#undef EDGE
+#undef FULLQ
+    *so = s;
+    *to = t;
+    return r < 0.0 ? 0.0 : r;
+}
+
+API double oracle_tri_closest(const double* v0, const double* v1, const double* v2, const double* q, double* st)
+{
+    return tri_closest(v0, v1, v2, q, st, st + 1);
+}
+
+/* For every point: smallest squared distance over all facets (first facet on exact ties), the
+ * facet and the closest point. */
+API void oracle_closest(const float* V, const int32_t* F, int64_t nF, const float* Q, int64_t nQ, double* d2,
+                        int32_t* facet, double* cp, int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+    for (int64_t i = 0; i < nQ; ++i) {
+        const double q[3] = {Q[3 * i], Q[3 * i + 1], Q[3 * i + 2]};
+        double best = INFINITY, bs = 0.0, bt = 0.0;
+        int32_t bf = -1;
+        for (int64_t f = 0; f < nF; ++f) {
+            double v[3][3];
+            for (int k = 0; k < 3; ++k)
+                for (int d = 0; d < 3; ++d) v[k][d] = V[3 * (int64_t)F[3 * f + k] + d];
+            double s, t;
+            const double r = tri_closest(v[0], v[1], v[2], q, &s, &t);
+            if (r < best) {
+                best = r;
+                bf = (int32_t)f;
+                bs = s;
+                bt = t;
+            }
+        }
+        d2[i] = best;
+        facet[i] = bf;
+        for (int d = 0; d < 3; ++d) {
+            if (bf < 0) {
+                cp[3 * i + d] = NAN;
+                continue;
+            }
+            const double a = V[3 * (int64_t)F[3 * bf] + d], b = V[3 * (int64_t)F[3 * bf + 1] + d],
+                         c = V[3 * (int64_t)F[3 * bf + 2] + d];
+            cp[3 * i + d] = a + bs * (b - a) + bt * (c - a);
+        }
+    }
 }

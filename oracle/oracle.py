@@ -70,6 +70,9 @@ def lib():
                                 C.c_float, FP, I32P, C.c_int]
         L.inside_from_omega.argtypes = [FP, C.c_int64, U8P]
         L.lbvh_sorted_order.argtypes = [C.c_int32, FP, I64P, I32P, I64P, I64P, U64P]
+        L.oracle_tri_closest.restype = C.c_double
+        L.oracle_tri_closest.argtypes = [DP, DP, DP, DP, DP]
+        L.oracle_closest.argtypes = [FP, I32P, C.c_int64, FP, C.c_int64, DP, I32P, DP, C.c_int]
         _lib = L
     return _lib
 
@@ -93,6 +96,25 @@ def bruteforce(V, F, Q, nthreads: int = 0) -> np.ndarray:
     lib().oracle_bruteforce(_p(V, FP), V.shape[0], _p(F, I32P), F.shape[0], _p(Q, FP), Q.shape[0],
                             _p(out, DP), nthreads)
     return out
+
+
+def tri_closest(v0, v1, v2, q):
+    """(squared distance, (s, t)) from q to triangle (v0, v1, v2); closest = v0 + s e0 + t e1."""
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (v0, v1, v2, q)]
+    st = np.zeros(2)
+    d = lib().oracle_tri_closest(*[_p(x, DP) for x in a], _p(st, DP))
+    return d, st
+
+
+def closest(V, F, Q, nthreads: int = 0):
+    """Brute-force closest point: (squared distance, facet, closest point) per point."""
+    V, F, Q = _f32(V), _i32(F), _f32(Q)
+    d2 = np.empty(Q.shape[0], np.float64)
+    fi = np.empty(Q.shape[0], np.int32)
+    cp = np.empty((Q.shape[0], 3), np.float64)
+    lib().oracle_closest(_p(V, FP), _p(F, I32P), F.shape[0], _p(Q, FP), Q.shape[0], _p(d2, DP), _p(fi, I32P),
+                         _p(cp, DP), nthreads)
+    return d2, fi, cp
 
 
 def tri_f64(v0, v1, v2, q) -> float:
