@@ -166,8 +166,6 @@ __device__ __forceinline__ float far_field_r(const SlotRegs& r, float dx, float 
     return fmaf(o0 + sw(r, FWN_W_TR), inv3, fmaf(q1, inv5, q2 * inv7));
 }
 
-// ---- wave stack in VGPRs: entry i sits in lane (i & 63) of bank (i >> 6) -----------------
-// Named registers (no arrays) so the banks stay in VGPRs rather than private memory.
 // One batch of up to 64 points (one per lane) of one mesh, walked through that mesh's tree.
 template <bool STATS>
 __device__ __forceinline__ void traverse_batch(
