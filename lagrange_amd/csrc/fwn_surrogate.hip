@@ -366,15 +366,18 @@ __global__ __launch_bounds__(256) void k_fallback_flags(const float* __restrict_
 // ============================================================================================
 // inference: encode (+ distance-bound flag), then the fused 4-layer MFMA MLP
 // ============================================================================================
-__global__ __launch_bounds__(256) void k_sg_encode(const float* __restrict__ P, int64_t n, GridParams g,
-                                                   const __half* __restrict__ table, int R,
+// Row i encodes point perm[i] (or i without a permutation): with the points in Morton order the
+// 64 lanes of a wave gather from neighbouring cells, so the coarse levels hit the same cache lines.
+__global__ __launch_bounds__(256) void k_sg_encode(const float* __restrict__ P, const uint32_t* __restrict__ perm,
+                                                   int64_t n, GridParams g, const __half* __restrict__ table, int R,
                                                    const uint32_t* __restrict__ bits, uint16_t* __restrict__ X0,
                                                    uint8_t* __restrict__ fb)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const float px = (P[3 * i] - g.lo[0]) * g.inv[0], py = (P[3 * i + 1] - g.lo[1]) * g.inv[1],
-                pz = (P[3 * i + 2] - g.lo[2]) * g.inv[2];
+    const int64_t j = perm ? (int64_t)perm[i] : i;
+    const float px = (P[3 * j] - g.lo[0]) * g.inv[0], py = (P[3 * j + 1] - g.lo[1]) * g.inv[1],
+                pz = (P[3 * j + 2] - g.lo[2]) * g.inv[2];
     fb[i] = near_cell(bits, R, px, py, pz) ? 1 : 0;
     uint32_t packed[16];
     for (int l = 0; l < kLevelsMax; ++l) {
@@ -392,6 +395,18 @@ __global__ __launch_bounds__(256) void k_sg_encode(const float* __restrict__ P, 
     uint4* dst = reinterpret_cast<uint4*>(X0 + (size_t)i * 32);
     for (int q = 0; q < 4; ++q)
         dst[q] = make_uint4(packed[4 * q], packed[4 * q + 1], packed[4 * q + 2], packed[4 * q + 3]);
+}
+
+// results of sorted rows back to input order
+__global__ __launch_bounds__(256) void k_sg_unpermute(const float* __restrict__ pred_s, const uint8_t* __restrict__ fb_s,
+                                                      const uint32_t* __restrict__ perm, int64_t n,
+                                                      float* __restrict__ pred, uint8_t* __restrict__ fb)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = perm[i];
+    pred[j] = pred_s[i];
+    fb[j] = fb_s[i];
 }
 
 // Physical k index -> logical input feature: k-step s, lane half h, element e.  With this order
@@ -422,6 +437,7 @@ __global__ void k_pack_frags(const uint16_t* __restrict__ mlp_bf, const int64_t*
 }
 
 constexpr int IW = 4;                 // waves per inference workgroup (one per SIMD: 512 registers per wave)
+constexpr int64_t kSortMin = 1 << 16; // batches at least this large are encoded in Morton order
 constexpr int kFrags = 8 + 3 * 32;    // A fragments of the four hidden layers
 
 __device__ __forceinline__ uint32_t pack2(float a, float b) { return (uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16); }
@@ -555,6 +571,7 @@ struct Surrogate {
     bool frags_ok = false;
     // activations / scratch (grown on demand)
     DBuf P, y, X0, Xh[4], dP, dQ, dX0, pred, dpred, parts, loss, fb;
+    DBuf pred_s, fb_s; // inference results in Morton order, before the scatter back
     int adam_t = 0;
     int n_cu = 0;
     hipEvent_t ev[6] = {};
@@ -892,8 +909,18 @@ void surrogate_predict(Surrogate& S, const float* P, int64_t n, float* pred_out,
     if (!S.ev[0])
         for (auto& ev : S.ev) FWN_HIP(hipEventCreate(&ev));
     FWN_HIP(hipEventRecord(S.ev[0], s));
-    hipLaunchKernelGGL(k_sg_encode, dim3(grid_blocks(n, 256)), dim3(256), 0, s, P, n, S.grid, S.table_h.as<__half>(),
-                       S.R, S.bits.as<uint32_t>(), S.X0.as<uint16_t>(), fb);
+    // large batches run in Morton order (the exact engine's point sort), then scatter back
+    const uint32_t* perm = n >= kSortMin ? sort_points_single(*S.exact, P, n) : nullptr;
+    float* pred_k = pred_out;
+    uint8_t* fb_k = fb;
+    if (perm) {
+        S.pred_s.reserve(sizeof(float) * n);
+        S.fb_s.reserve(n);
+        pred_k = S.pred_s.as<float>();
+        fb_k = S.fb_s.as<uint8_t>();
+    }
+    hipLaunchKernelGGL(k_sg_encode, dim3(grid_blocks(n, 256)), dim3(256), 0, s, P, perm, n, S.grid,
+                       S.table_h.as<__half>(), S.R, S.bits.as<uint32_t>(), S.X0.as<uint16_t>(), fb_k);
     FWN_LAUNCH_CHECK("k_sg_encode");
     FWN_HIP(hipEventRecord(S.ev[1], s));
     if (S.n_cu == 0) {
@@ -905,14 +932,21 @@ void surrogate_predict(Surrogate& S, const float* P, int64_t n, float* pred_out,
     const int64_t blocks = std::min<int64_t>((ntiles + IW - 1) / IW, (int64_t)S.n_cu);
     hipLaunchKernelGGL(k_sg_mlp, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(IW * 64), 0, s,
                        S.X0.as<uint16_t>(), n, S.frags.as<uint16_t>(), S.mlp.as<float>(), S.off_b[0], S.off_b[1],
-                       S.off_b[2], S.off_b[3], S.off_w[4], S.off_b[4], S.opt.band, pred_out, fb);
+                       S.off_b[2], S.off_b[3], S.off_w[4], S.off_b[4], S.opt.band, pred_k, fb_k);
     FWN_LAUNCH_CHECK("k_sg_mlp");
     FWN_HIP(hipEventRecord(S.ev[2], s));
-    FWN_HIP(hipEventSynchronize(S.ev[2]));
-    float a = 0.0f, b = 0.0f;
+    if (perm) {
+        hipLaunchKernelGGL(k_sg_unpermute, dim3(grid_blocks(n, 256)), dim3(256), 0, s, pred_k, fb_k, perm, n,
+                           pred_out, fb);
+        FWN_LAUNCH_CHECK("k_sg_unpermute");
+    }
+    FWN_HIP(hipEventRecord(S.ev[3], s));
+    FWN_HIP(hipEventSynchronize(S.ev[3]));
+    float a = 0.0f, b = 0.0f, c = 0.0f;
     FWN_HIP(hipEventElapsedTime(&a, S.ev[0], S.ev[1]));
     FWN_HIP(hipEventElapsedTime(&b, S.ev[1], S.ev[2]));
-    S.encode_ms = a;
+    FWN_HIP(hipEventElapsedTime(&c, S.ev[2], S.ev[3]));
+    S.encode_ms = a + c; // sort + encode + scatter back
     S.mlp_ms = b;
 }
 

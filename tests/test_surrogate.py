@@ -132,3 +132,23 @@ def test_trained_surrogate_gain_and_exact_fallback():
     assert nf / q.shape[0] <= 0.35 and nf < nf_u
     assert mis <= 1e-4
     assert info["last_loss"] < 0.05
+
+
+@pytest.mark.gpu
+def test_sorted_inference_equals_unsorted_chunks():
+    """Large batches are encoded in Morton order and scattered back: per point the result is the
+    same as for small batches, which run in input order."""
+    from lagrange_amd import workloads as W
+    from lagrange_amd.surrogate import Surrogate
+    from lagrange_amd.winding import FastWindingNumber
+    V, F = W.displaced_sphere(16, 2, delete_fraction=0.05).f32()
+    e = FastWindingNumber((V, F))
+    s = Surrogate(e, n_samples=200_000, steps=50, grid_res=128, batch=65536)
+    s.train()
+    q = W.uniform_queries(V.min(0), V.max(0), 150_000, 33)
+    big, fb_big = s.predict(q)
+    parts = [s.predict(q[i:i + 40_000]) for i in range(0, q.shape[0], 40_000)]
+    np.testing.assert_array_equal(big, np.concatenate([p[0] for p in parts]))
+    np.testing.assert_array_equal(fb_big, np.concatenate([p[1] for p in parts]))
+    w, ins, nf = s.query(q)
+    assert nf == int(fb_big.sum())
