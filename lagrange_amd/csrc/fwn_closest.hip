@@ -240,6 +240,7 @@ __device__ __forceinline__ float wave_min_f(float v)
     return v;
 }
 
+template <int NB>
 __global__ __launch_bounds__(64) void k_closest(const uint32_t* __restrict__ slots, const float* __restrict__ boxes,
                                                 int width, const uint32_t* __restrict__ tris, int32_t root,
                                                 int32_t root_ns, const float* __restrict__ Q,
@@ -258,15 +259,15 @@ __global__ __launch_bounds__(64) void k_closest(const uint32_t* __restrict__ slo
     double best = INFINITY, bs = 0.0, bt = 0.0;
     int32_t btri = -1;
     if (root >= 0) {
-        WaveStack S;
-        stack_push(S, 0, lane, ((uint32_t)root << 4) | (uint32_t)root_ns, __ballot(valid));
+        WaveStack<NB> S;
+        S.push(0, lane, ((uint32_t)root << 4) | (uint32_t)root_ns, __ballot(valid));
         int sp = 1;
         bool overflow = false;
         while (sp > 0) {
             --sp;
             uint32_t e;
             uint64_t mk;
-            stack_pop(S, sp, e, mk);
+            S.pop(sp, e, mk);
             const bool act = (mk >> lane) & 1ull;
             const int64_t node = e >> 4;
             const int ns = (int)(e & 15u);
@@ -329,9 +330,9 @@ __global__ __launch_bounds__(64) void k_closest(const uint32_t* __restrict__ slo
 #pragma unroll
                 for (int s = 0; s < kWidth; ++s) {
                     if (s == sel) {
-                        if (sp < kStack) {
+                        if (sp < WaveStack<NB>::kCap) {
                             const uint32_t cns = nb[s * kSlotWords + FWN_W_CHILD_NS];
-                            stack_push(S, sp, lane, ((uint32_t)child[s] << 4) | cns, pm[s]);
+                            S.push(sp, lane, ((uint32_t)child[s] << 4) | cns, pm[s]);
                             ++sp;
                         } else {
                             overflow = true;
@@ -368,8 +369,8 @@ void run_closest(Engine& e, const float* Q, int64_t nQ, double* d2, int32_t* tri
     if (nQ < 0) fail(FWN_INVALID_ARGUMENT, "negative point count");
     if (nQ == 0) return;
     if (!Q || !d2 || !tri) fail(FWN_INVALID_ARGUMENT, "NULL points or output");
-    if (1 + (int64_t)(e.width - 1) * e.max_depth > kStack)
-        fail(FWN_LIMIT_EXCEEDED, "tree deeper than the traversal stack supports");
+    const int banks = stack_banks(e.width, e.max_depth);
+    if (banks == 0) fail(FWN_LIMIT_EXCEEDED, "tree deeper than the traversal stack supports");
     hipStream_t s = e.stream;
     const float* dQ = Q;
     double* dD = d2;
@@ -394,8 +395,14 @@ void run_closest(Engine& e, const float* Q, int64_t nQ, double* d2, int32_t* tri
     FWN_HIP(hipMemsetAsync(dflag, 0, sizeof(int32_t), s));
     const int64_t nb = (nQ + 63) / 64;
     if (nb > INT32_MAX) fail(FWN_LIMIT_EXCEEDED, "too many point batches");
-    hipLaunchKernelGGL(k_closest, dim3((unsigned)nb), dim3(64), 0, s, e.slots.as<uint32_t>(), e.boxes.as<float>(),
-                       e.width, e.tris.as<uint32_t>(), e.h_roots[0], e.h_roots[1], dQ, perm, nQ, dD, dT, dC, dflag);
+    if (banks == 1)
+        hipLaunchKernelGGL(k_closest<1>, dim3((unsigned)nb), dim3(64), 0, s, e.slots.as<uint32_t>(),
+                           e.boxes.as<float>(), e.width, e.tris.as<uint32_t>(), e.h_roots[0], e.h_roots[1], dQ, perm,
+                           nQ, dD, dT, dC, dflag);
+    else
+        hipLaunchKernelGGL(k_closest<3>, dim3((unsigned)nb), dim3(64), 0, s, e.slots.as<uint32_t>(),
+                           e.boxes.as<float>(), e.width, e.tris.as<uint32_t>(), e.h_roots[0], e.h_roots[1], dQ, perm,
+                           nQ, dD, dT, dC, dflag);
     FWN_LAUNCH_CHECK("k_closest");
     int32_t h_flag = 0;
     FWN_HIP(hipMemcpyAsync(&h_flag, dflag, sizeof(int32_t), hipMemcpyDeviceToHost, s));

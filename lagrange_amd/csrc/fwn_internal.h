@@ -223,43 +223,74 @@ __host__ __device__ inline float ord2f(int32_t i)
 }
 
 // Wave-private traversal stack of (node entry, 64-bit lane mask), one entry per lane and bank:
-// entry k lives in lane k % 64 of bank k / 64 (3 banks = kStack entries), so a push or pop is a
-// select / v_readlane on uniform indices and the stack never touches scratch memory.
-struct WaveStack {
-    uint32_t n0 = 0, n1 = 0, n2 = 0, l0 = 0, l1 = 0, l2 = 0, h0 = 0, h1 = 0, h2 = 0;
+// entry k lives in lane k % 64 of bank k / 64.  NB = 1 (64 entries, enough for trees with
+// 1 + (width - 1) * depth <= 64) or 3 (kStack entries).  A push is a lane select, a pop is
+// v_readlane; every bank is updated / read under uniform predicates with no branches, so nothing
+// invites the optimiser to turn the banks into an indexed private array in scratch memory.  A
+// push past the capacity writes no lane (callers report it as an overflow).
+template <int NB>
+struct WaveStack;
+
+template <>
+struct WaveStack<1> {
+    static constexpr int kCap = 64;
+    uint32_t n0 = 0, l0 = 0, h0 = 0;
+    __device__ __forceinline__ void push(int sp, int lane, uint32_t e, uint64_t mk)
+    {
+        const bool me = lane == sp;
+        n0 = me ? e : n0;
+        l0 = me ? (uint32_t)mk : l0;
+        h0 = me ? (uint32_t)(mk >> 32) : h0;
+    }
+    __device__ __forceinline__ void pop(int sp, uint32_t& e, uint64_t& mk) const
+    {
+        const int l = sp & 63;
+        e = __builtin_amdgcn_readlane(n0, l);
+        const uint32_t lo = __builtin_amdgcn_readlane(l0, l), hi = __builtin_amdgcn_readlane(h0, l);
+        mk = ((uint64_t)hi << 32) | lo;
+    }
 };
 
-// Branch-free: every bank is updated / read under uniform predicates, so nothing invites the
-// optimiser to turn the banks into an indexed private array.
-__device__ __forceinline__ void stack_push(WaveStack& S, int sp, int lane, uint32_t e, uint64_t mk)
-{
-    const bool me = lane == (sp & 63);
-    const bool b0 = me && sp < 64, b1 = me && sp >= 64 && sp < 128, b2 = me && sp >= 128;
-    const uint32_t mlo = (uint32_t)mk, mhi = (uint32_t)(mk >> 32);
-    S.n0 = b0 ? e : S.n0;
-    S.l0 = b0 ? mlo : S.l0;
-    S.h0 = b0 ? mhi : S.h0;
-    S.n1 = b1 ? e : S.n1;
-    S.l1 = b1 ? mlo : S.l1;
-    S.h1 = b1 ? mhi : S.h1;
-    S.n2 = b2 ? e : S.n2;
-    S.l2 = b2 ? mlo : S.l2;
-    S.h2 = b2 ? mhi : S.h2;
-}
+template <>
+struct WaveStack<3> {
+    static constexpr int kCap = kStack;
+    uint32_t n0 = 0, n1 = 0, n2 = 0, l0 = 0, l1 = 0, l2 = 0, h0 = 0, h1 = 0, h2 = 0;
+    __device__ __forceinline__ void push(int sp, int lane, uint32_t e, uint64_t mk)
+    {
+        const bool me = lane == (sp & 63);
+        const bool b0 = me && sp < 64, b1 = me && sp >= 64 && sp < 128, b2 = me && sp >= 128 && sp < kStack;
+        const uint32_t mlo = (uint32_t)mk, mhi = (uint32_t)(mk >> 32);
+        n0 = b0 ? e : n0;
+        l0 = b0 ? mlo : l0;
+        h0 = b0 ? mhi : h0;
+        n1 = b1 ? e : n1;
+        l1 = b1 ? mlo : l1;
+        h1 = b1 ? mhi : h1;
+        n2 = b2 ? e : n2;
+        l2 = b2 ? mlo : l2;
+        h2 = b2 ? mhi : h2;
+    }
+    __device__ __forceinline__ void pop(int sp, uint32_t& e, uint64_t& mk) const
+    {
+        const int l = sp & 63;
+        const uint32_t a0 = __builtin_amdgcn_readlane(n0, l), a1 = __builtin_amdgcn_readlane(n1, l),
+                       a2 = __builtin_amdgcn_readlane(n2, l);
+        const uint32_t c0 = __builtin_amdgcn_readlane(l0, l), c1 = __builtin_amdgcn_readlane(l1, l),
+                       c2 = __builtin_amdgcn_readlane(l2, l);
+        const uint32_t d0 = __builtin_amdgcn_readlane(h0, l), d1 = __builtin_amdgcn_readlane(h1, l),
+                       d2 = __builtin_amdgcn_readlane(h2, l);
+        const uint32_t lo = sp < 64 ? c0 : (sp < 128 ? c1 : c2);
+        const uint32_t hi = sp < 64 ? d0 : (sp < 128 ? d1 : d2);
+        e = sp < 64 ? a0 : (sp < 128 ? a1 : a2);
+        mk = ((uint64_t)hi << 32) | lo;
+    }
+};
 
-__device__ __forceinline__ void stack_pop(const WaveStack& S, int sp, uint32_t& e, uint64_t& mk)
+// stack banks a tree of this width and depth needs (1 or 3), 0 if it is too deep
+inline int stack_banks(int width, int depth)
 {
-    const int l = sp & 63;
-    const uint32_t a0 = __builtin_amdgcn_readlane(S.n0, l), a1 = __builtin_amdgcn_readlane(S.n1, l),
-                   a2 = __builtin_amdgcn_readlane(S.n2, l);
-    const uint32_t c0 = __builtin_amdgcn_readlane(S.l0, l), c1 = __builtin_amdgcn_readlane(S.l1, l),
-                   c2 = __builtin_amdgcn_readlane(S.l2, l);
-    const uint32_t d0 = __builtin_amdgcn_readlane(S.h0, l), d1 = __builtin_amdgcn_readlane(S.h1, l),
-                   d2 = __builtin_amdgcn_readlane(S.h2, l);
-    const uint32_t lo = sp < 64 ? c0 : (sp < 128 ? c1 : c2);
-    const uint32_t hi = sp < 64 ? d0 : (sp < 128 ? d1 : d2);
-    e = sp < 64 ? a0 : (sp < 128 ? a1 : a2);
-    mk = ((uint64_t)hi << 32) | lo;
+    const int need = 1 + (width - 1) * depth;
+    return need <= 64 ? 1 : (need <= kStack ? 3 : 0);
 }
 
 // XCD-aware block remap: hardware hands consecutive workgroups to the 8 XCDs round-robin;

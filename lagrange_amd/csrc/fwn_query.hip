@@ -167,7 +167,7 @@ __device__ __forceinline__ float far_field_r(const SlotRegs& r, float dx, float 
 }
 
 // One batch of up to 64 points (one per lane) of one mesh, walked through that mesh's tree.
-template <bool STATS>
+template <bool STATS, int NB>
 __device__ __forceinline__ void traverse_batch(
     int64_t batch, const uint32_t* __restrict__ slots, int width, const uint32_t* __restrict__ tris,
     const int32_t* __restrict__ roots, const float* __restrict__ minfo, const float* __restrict__ Q,
@@ -198,15 +198,15 @@ __device__ __forceinline__ void traverse_batch(
     float om = 0.0f;
     int32_t n_test = 0, n_far = 0, n_t32 = 0, n_t64 = 0;
     if (root >= 0) {
-        WaveStack S;
-        stack_push(S, 0, lane, ((uint32_t)root << 4) | (uint32_t)root_ns, __ballot(valid));
+        WaveStack<NB> S;
+        S.push(0, lane, ((uint32_t)root << 4) | (uint32_t)root_ns, __ballot(valid));
         int sp = 1;
         bool overflow = false;
         while (sp > 0) {
             --sp;
             uint32_t e;
             uint64_t mk;
-            stack_pop(S, sp, e, mk);
+            S.pop(sp, e, mk);
             const bool act = (mk >> lane) & 1ull;
             const uint32_t* nb = slots + (size_t)(e >> 4) * (size_t)width * kSlotWords;
             const int ns = (int)(e & 15u);
@@ -218,12 +218,14 @@ __device__ __forceinline__ void traverse_batch(
                 const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                 const float r2 = sw(cur, FWN_W_R2);
                 const bool far = d2 > beta2 * r2;
+                // near lanes: active and not far (scalar mask arithmetic on the compare's ballot,
+                // taken before the divergent far-field branch)
+                const uint64_t nearm = mk & ~__ballot(far);
                 if (STATS) n_test += act ? 1 : 0;
                 if (act && far) {
                     om += far_field_r(cur, dx, dy, dz, d2);
                     if (STATS) ++n_far;
                 }
-                const uint64_t nearm = __ballot(act && !far);
                 if (nearm != 0ull) {
                     const int32_t child = si(cur, FWN_W_CHILD);
                     if (child < 0) {
@@ -263,12 +265,13 @@ __device__ __forceinline__ void traverse_batch(
                                 if (STATS) n_t64 += tcnt;
                             }
                         }
-                    } else if (sp < kStack) {
-                        const uint32_t cns = (uint32_t)si(cur, FWN_W_CHILD_NS);
-                        stack_push(S, sp, lane, ((uint32_t)child << 4) | cns, nearm);
-                        ++sp;
                     } else {
-                        overflow = true;
+                        // the host only launches trees whose stack bound 1 + (width - 1) depth fits;
+                        // past the capacity a push writes no lane and the overflow is reported
+                        overflow |= sp >= WaveStack<NB>::kCap;
+                        const uint32_t cns = (uint32_t)si(cur, FWN_W_CHILD_NS);
+                        S.push(sp, lane, ((uint32_t)child << 4) | cns, nearm);
+                        ++sp;
                     }
                 }
                 cur = nxt;
@@ -291,7 +294,7 @@ __device__ __forceinline__ void traverse_batch(
 // Grid launch: one wave per batch, XCD-aware block order.  Persistent launch: a grid sized to
 // the device's resident waves; each wave takes batches from a work counter until none is left
 // (the batch numbering puts the meshes with the most triangles first).
-template <bool STATS, int WAVES, bool PERSIST>
+template <bool STATS, int WAVES, bool PERSIST, int NB>
 __global__ __launch_bounds__(WAVES * 64) void k_traverse(
     const uint32_t* __restrict__ slots, int width, const uint32_t* __restrict__ tris,
     const int32_t* __restrict__ roots, const float* __restrict__ minfo, const float* __restrict__ Q,
@@ -303,7 +306,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_traverse(
     if (!PERSIST) {
         const int64_t batch = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * WAVES + (threadIdx.x >> 6);
         if (batch >= n_batches) return; // whole wave
-        traverse_batch<STATS>(batch, slots, width, tris, roots, minfo, Q, perm, q_off, b_off, mesh_order, M, beta2,
+        traverse_batch<STATS, NB>(batch, slots, width, tris, roots, minfo, Q, perm, q_off, b_off, mesh_order, M, beta2,
                               omega, inside, stats, items, n_items, item_cap, flag);
     } else {
         for (;;) {
@@ -311,7 +314,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_traverse(
             if ((threadIdx.x & 63) == 0) b = atomicAdd(work, 1u);
             b = __builtin_amdgcn_readfirstlane(b);
             if ((int64_t)b >= n_batches) break; // every wave leaves once the counter passes the end
-            traverse_batch<STATS>((int64_t)b, slots, width, tris, roots, minfo, Q, perm, q_off, b_off, mesh_order, M,
+            traverse_batch<STATS, NB>((int64_t)b, slots, width, tris, roots, minfo, Q, perm, q_off, b_off, mesh_order, M,
                                   beta2, omega, inside, stats, items, n_items, item_cap, flag);
         }
     }
@@ -384,16 +387,21 @@ __global__ void k_zero3(int32_t* a)
     a[2] = 0;
 }
 
-template <bool ST, int WV, bool PE>
+template <bool ST, int WV, bool PE, int NB>
 void* traverse_fn()
 {
-    return reinterpret_cast<void*>(&k_traverse<ST, WV, PE>);
+    return reinterpret_cast<void*>(&k_traverse<ST, WV, PE, NB>);
 }
 
+// banks: traversal stack banks the tree needs (1: 64 entries, 3: kStack entries)
 template <typename... Args>
-void launch_traverse(bool stats, int waves, bool persist, unsigned nblk, hipStream_t s, Args... args)
+void launch_traverse(bool stats, int waves, bool persist, int banks, unsigned nblk, hipStream_t s, Args... args)
 {
-#define FWN_TRAV(ST, WV, PE) hipLaunchKernelGGL((k_traverse<ST, WV, PE>), dim3(nblk), dim3(WV * 64), 0, s, args...)
+#define FWN_TRAV(ST, WV, PE)                                                                                  \
+    do {                                                                                                      \
+        if (banks == 1) hipLaunchKernelGGL((k_traverse<ST, WV, PE, 1>), dim3(nblk), dim3(WV * 64), 0, s, args...); \
+        else hipLaunchKernelGGL((k_traverse<ST, WV, PE, 3>), dim3(nblk), dim3(WV * 64), 0, s, args...);        \
+    } while (0)
     if (stats) {
         FWN_TRAV(true, 1, false);
         return;
@@ -412,14 +420,15 @@ void launch_traverse(bool stats, int waves, bool persist, unsigned nblk, hipStre
 }
 
 // resident workgroups per CU of a traversal variant (persistent grid size)
-int resident_blocks(bool persist, int waves)
+int resident_blocks(bool persist, int waves, int banks)
 {
     const void* f = nullptr;
+    const bool one = banks == 1;
     switch (waves * 2 + (persist ? 1 : 0)) {
-    case 3: f = traverse_fn<false, 1, true>(); break;
-    case 5: f = traverse_fn<false, 2, true>(); break;
-    case 9: f = traverse_fn<false, 4, true>(); break;
-    case 17: f = traverse_fn<false, 8, true>(); break;
+    case 3: f = one ? traverse_fn<false, 1, true, 1>() : traverse_fn<false, 1, true, 3>(); break;
+    case 5: f = one ? traverse_fn<false, 2, true, 1>() : traverse_fn<false, 2, true, 3>(); break;
+    case 9: f = one ? traverse_fn<false, 4, true, 1>() : traverse_fn<false, 4, true, 3>(); break;
+    case 17: f = one ? traverse_fn<false, 8, true, 1>() : traverse_fn<false, 8, true, 3>(); break;
     default: return 1;
     }
     int nb = 0;
@@ -449,8 +458,8 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
     if (nQ < 0) fail(FWN_INVALID_ARGUMENT, "negative point count");
     if (nQ == 0) return;
     if (nQ > (int64_t)UINT32_MAX / 2) fail(FWN_LIMIT_EXCEEDED, "more than 2^31 points in one call");
-    if (1 + (int64_t)(e.width - 1) * e.max_depth > kStack)
-        fail(FWN_LIMIT_EXCEEDED, "tree deeper than the traversal stack supports");
+    const int banks = stack_banks(e.width, e.max_depth);
+    if (banks == 0) fail(FWN_LIMIT_EXCEEDED, "tree deeper than the traversal stack supports");
 
     // batches of 64 points, numbered over the meshes in decreasing triangle count so that a
     // persistent launch starts the deepest trees first
@@ -531,7 +540,7 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
             FWN_HIP(hipGetDeviceProperties(&prop, e.device));
             e.n_cu = prop.multiProcessorCount;
         }
-        nblk = std::min<int64_t>(nblk, (int64_t)e.n_cu * resident_blocks(true, waves));
+        nblk = std::min<int64_t>(nblk, (int64_t)e.n_cu * resident_blocks(true, waves, banks));
     }
     if (nblk > INT32_MAX) fail(FWN_LIMIT_EXCEEDED, "too many point batches");
     const float beta2 = e.opt.accuracy_scale * e.opt.accuracy_scale;
@@ -542,7 +551,7 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
         hipLaunchKernelGGL(k_zero3, dim3(1), dim3(1), 0, s, dflag);
         FWN_HIP(hipEventRecord(e.ev[1], s));
         const uint32_t cap = (uint32_t)e.item_cap;
-        launch_traverse(stats != nullptr, waves, persist, (unsigned)nblk, s, e.slots.as<uint32_t>(), e.width,
+        launch_traverse(stats != nullptr, waves, persist, banks, (unsigned)nblk, s, e.slots.as<uint32_t>(), e.width,
                         e.tris.as<uint32_t>(), e.roots.as<int32_t>(), e.mesh_info.as<float>(), dQ, vp,
                         e.q_off.as<int64_t>(), e.q_boff.as<int64_t>(), e.q_order.as<int32_t>(), M, nb, beta2, dOm,
                         dIn, dSt, e.q_items.as<uint32_t>(), dnitems, cap, dflag, dwork);
