@@ -108,7 +108,8 @@ enum {
     FWN_W_A = 8,  /* 6 words */
     FWN_W_B = 14, /* 10 words */
     FWN_W_C = 24, /* 3 words */
-    FWN_W_CHILD = 27, FWN_W_FIRST = 28, FWN_W_COUNT = 29, FWN_W_CHILD_NS = 30
+    FWN_W_CHILD = 27, FWN_W_FIRST = 28, FWN_W_COUNT = 29, FWN_W_CHILD_NS = 30,
+    FWN_W_THR = 31 /* beta^2 r^2 in float: the far-field threshold for the engine's accuracy scale */
 };
 
 typedef struct fwn_tree_info {

@@ -461,7 +461,7 @@ __global__ __launch_bounds__(BT) void k_write_slots(int64_t W, int width, const 
                                                     const int32_t* __restrict__ child_wide,
                                                     const double* __restrict__ rec,
                                                     const int32_t* __restrict__ first,
-                                                    const int32_t* __restrict__ last, int64_t n,
+                                                    const int32_t* __restrict__ last, int64_t n, float beta2,
                                                     uint32_t* __restrict__ slots, float* __restrict__ boxes)
 {
     int64_t idx = (int64_t)blockIdx.x * BT + threadIdx.x;
@@ -520,7 +520,7 @@ __global__ __launch_bounds__(BT) void k_write_slots(int64_t W, int width, const 
     w[FWN_W_FIRST] = (uint32_t)(bin_leaf ? (int32_t)(u - (n - 1)) : first[u]);
     w[FWN_W_COUNT] = (uint32_t)(bin_leaf ? 1 : last[u] - first[u] + 1);
     w[FWN_W_CHILD_NS] = (uint32_t)(cw >= 0 ? wide_ns[cw] : 0);
-    w[31] = 0u;
+    w[FWN_W_THR] = __float_as_uint(beta2 * f[3]); // the query's far-field threshold
 }
 
 } // namespace
@@ -720,7 +720,7 @@ void build_engine(Engine& e, int32_t M, const float* V_host, const int64_t* v_of
     e.boxes.alloc(sizeof(float) * width * 6 * (size_t)std::max<int64_t>(W, 1));
     hipLaunchKernelGGL(k_write_slots, dim3(grid_blocks(W * width, BT)), dim3(BT), 0, s, W, width, wsb.as<int32_t>(),
                        wns.as<int32_t>(), cw.as<int32_t>(), rec.as<double>(), first.as<int32_t>(), last.as<int32_t>(),
-                       n, e.slots.as<uint32_t>(), e.boxes.as<float>());
+                       n, e.opt.accuracy_scale * e.opt.accuracy_scale, e.slots.as<uint32_t>(), e.boxes.as<float>());
     FWN_LAUNCH_CHECK("k_write_slots");
     std::vector<int32_t> ns_root(level0.size());
     if (!level0.empty())

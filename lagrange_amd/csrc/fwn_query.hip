@@ -108,27 +108,6 @@ __device__ __forceinline__ double tri_f64(const float* __restrict__ t, float qx,
     return 2.0 * atan2(num, den);
 }
 
-// order-2 far field of one slot at offset d = p - q (fwn.h slot words)
-__device__ __forceinline__ float far_field(const float* __restrict__ w, float dx, float dy, float dz, float d2)
-{
-    float inv = __builtin_amdgcn_rsqf(d2);
-    float inv2 = inv * inv;
-    float inv3 = inv2 * inv;
-    float inv5 = inv3 * inv2;
-    float inv7 = inv5 * inv2;
-    float o0 = fmaf(dx, w[FWN_W_NX], fmaf(dy, w[FWN_W_NY], dz * w[FWN_W_NZ]));
-    float xx = dx * dx, yy = dy * dy, zz = dz * dz;
-    float xy = dx * dy, yz = dy * dz, zx = dz * dx;
-    const float* A = w + FWN_W_A;
-    float q1 = fmaf(A[0], xx, fmaf(A[1], yy, fmaf(A[2], zz, fmaf(A[3], xy, fmaf(A[4], yz, A[5] * zx)))));
-    q1 = fmaf(w[FWN_W_C], dx, fmaf(w[FWN_W_C + 1], dy, fmaf(w[FWN_W_C + 2], dz, q1)));
-    const float* B = w + FWN_W_B;
-    float q2 = fmaf(B[0], xx * dx, fmaf(B[1], yy * dy, fmaf(B[2], zz * dz,
-               fmaf(B[3], xx * dy, fmaf(B[4], xx * dz, fmaf(B[5], yy * dx,
-               fmaf(B[6], yy * dz, fmaf(B[7], zz * dx, fmaf(B[8], zz * dy, B[9] * (xy * dz))))))))));
-    return fmaf(o0 + w[FWN_W_TR], inv3, fmaf(q1, inv5, q2 * inv7));
-}
-
 // One slot's 32 words held in SGPRs (two 16-dword scalar loads).  The traversal issues the
 // next slot's loads before the current slot's arithmetic so the load latency overlaps it.
 typedef uint32_t v16u __attribute__((ext_vector_type(16)));
@@ -145,25 +124,30 @@ __device__ __forceinline__ SlotRegs load_slot(const uint32_t* __restrict__ p)
 __device__ __forceinline__ float sw(const SlotRegs& r, int k) { return __uint_as_float(k < 16 ? r.a[k] : r.b[k - 16]); }
 __device__ __forceinline__ int32_t si(const SlotRegs& r, int k) { return (int32_t)(k < 16 ? r.a[k] : r.b[k - 16]); }
 
-// order-2 far field of one slot held in registers
-__device__ __forceinline__ float far_field_r(const SlotRegs& r, float dx, float dy, float dz, float d2)
+// order-2 far field of one slot held in registers, added to om:
+//   omega += inv3 (o0 + inv2 (q1 + inv2 q2)),  inv = 1 / |d|,
+// o0 = d.N + tr M1, q1 = -3 d^T M1 d + C.d (C = -4.5 t), q2 = 7.5 S(d, d, d); q1 and q2 are
+// evaluated in Horner form over the packed coefficients (no monomials needed)
+__device__ __forceinline__ float far_field_acc(const SlotRegs& r, float dx, float dy, float dz, float d2, float om)
 {
-    float inv = __builtin_amdgcn_rsqf(d2);
-    float inv2 = inv * inv;
-    float inv3 = inv2 * inv;
-    float inv5 = inv3 * inv2;
-    float inv7 = inv5 * inv2;
-    float o0 = fmaf(dx, sw(r, FWN_W_NX), fmaf(dy, sw(r, FWN_W_NY), dz * sw(r, FWN_W_NZ)));
-    float xx = dx * dx, yy = dy * dy, zz = dz * dz;
-    float xy = dx * dy, yz = dy * dz, zx = dz * dx;
-    float q1 = fmaf(sw(r, FWN_W_A), xx, fmaf(sw(r, FWN_W_A + 1), yy, fmaf(sw(r, FWN_W_A + 2), zz,
-               fmaf(sw(r, FWN_W_A + 3), xy, fmaf(sw(r, FWN_W_A + 4), yz, sw(r, FWN_W_A + 5) * zx)))));
-    q1 = fmaf(sw(r, FWN_W_C), dx, fmaf(sw(r, FWN_W_C + 1), dy, fmaf(sw(r, FWN_W_C + 2), dz, q1)));
-    float q2 = fmaf(sw(r, FWN_W_B), xx * dx, fmaf(sw(r, FWN_W_B + 1), yy * dy, fmaf(sw(r, FWN_W_B + 2), zz * dz,
-               fmaf(sw(r, FWN_W_B + 3), xx * dy, fmaf(sw(r, FWN_W_B + 4), xx * dz, fmaf(sw(r, FWN_W_B + 5), yy * dx,
-               fmaf(sw(r, FWN_W_B + 6), yy * dz, fmaf(sw(r, FWN_W_B + 7), zz * dx,
-               fmaf(sw(r, FWN_W_B + 8), zz * dy, sw(r, FWN_W_B + 9) * (xy * dz))))))))));
-    return fmaf(o0 + sw(r, FWN_W_TR), inv3, fmaf(q1, inv5, q2 * inv7));
+    const float inv = __builtin_amdgcn_rsqf(d2);
+    const float inv2 = inv * inv;
+    const float inv3 = inv2 * inv;
+    const float o0 = fmaf(dx, sw(r, FWN_W_NX), fmaf(dy, sw(r, FWN_W_NY), fmaf(dz, sw(r, FWN_W_NZ), sw(r, FWN_W_TR))));
+    // A: xx yy zz xy yz zx;  C: x y z
+    const float a1 = fmaf(sw(r, FWN_W_A), dx, fmaf(sw(r, FWN_W_A + 3), dy, fmaf(sw(r, FWN_W_A + 5), dz, sw(r, FWN_W_C))));
+    const float b1 = fmaf(sw(r, FWN_W_A + 1), dy, fmaf(sw(r, FWN_W_A + 4), dz, sw(r, FWN_W_C + 1)));
+    const float c1 = fmaf(sw(r, FWN_W_A + 2), dz, sw(r, FWN_W_C + 2));
+    const float q1 = fmaf(dx, a1, fmaf(dy, b1, dz * c1));
+    // B: xxx yyy zzz xxy xxz yyx yyz zzx zzy xyz
+    const float ax = fmaf(sw(r, FWN_W_B), dx, fmaf(sw(r, FWN_W_B + 3), dy, sw(r, FWN_W_B + 4) * dz));
+    const float bx = fmaf(sw(r, FWN_W_B + 5), dy, sw(r, FWN_W_B + 9) * dz);
+    const float X = fmaf(dx, ax, fmaf(dy, bx, dz * (sw(r, FWN_W_B + 7) * dz)));
+    const float Y = fmaf(dy, fmaf(sw(r, FWN_W_B + 1), dy, sw(r, FWN_W_B + 6) * dz), dz * (sw(r, FWN_W_B + 8) * dz));
+    const float Z = dz * (dz * sw(r, FWN_W_B + 2));
+    const float q2 = fmaf(dx, X, fmaf(dy, Y, dz * Z));
+    const float t = fmaf(inv2, fmaf(inv2, q2, q1), o0);
+    return fmaf(t, inv3, om);
 }
 
 // One batch of up to 64 points (one per lane) of one mesh, walked through that mesh's tree.
@@ -210,20 +194,19 @@ __device__ __forceinline__ void traverse_batch(
             const bool act = (mk >> lane) & 1ull;
             const uint32_t* nb = slots + (size_t)(e >> 4) * (size_t)width * kSlotWords;
             const int ns = (int)(e & 15u);
-            SlotRegs cur = load_slot(nb);
-            for (int s = 0; s < ns; ++s) {
-                SlotRegs nxt = cur;
-                if (s + 1 < ns) nxt = load_slot(nb + (s + 1) * kSlotWords);
+            if (ns == 0) continue;
+            // one slot: test, far term, leaf triangles or push of the child
+            auto slot = [&](const SlotRegs& cur) __attribute__((always_inline)) {
                 const float dx = sw(cur, FWN_W_PX) - qx, dy = sw(cur, FWN_W_PY) - qy, dz = sw(cur, FWN_W_PZ) - qz;
                 const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                 const float r2 = sw(cur, FWN_W_R2);
-                const bool far = d2 > beta2 * r2;
+                const bool far = d2 > sw(cur, FWN_W_THR); // beta^2 r^2, written by the build
                 // near lanes: active and not far (scalar mask arithmetic on the compare's ballot,
                 // taken before the divergent far-field branch)
                 const uint64_t nearm = mk & ~__ballot(far);
                 if (STATS) n_test += act ? 1 : 0;
                 if (act && far) {
-                    om += far_field_r(cur, dx, dy, dz, d2);
+                    om = far_field_acc(cur, dx, dy, dz, d2, om);
                     if (STATS) ++n_far;
                 }
                 if (nearm != 0ull) {
@@ -274,6 +257,13 @@ __device__ __forceinline__ void traverse_batch(
                         ++sp;
                     }
                 }
+            };
+            // the next slot's scalar loads are issued before the current slot's arithmetic
+            SlotRegs cur = load_slot(nb);
+            for (int s = 0; s < ns; ++s) {
+                SlotRegs nxt = cur;
+                if (s + 1 < ns) nxt = load_slot(nb + (s + 1) * kSlotWords);
+                slot(cur);
                 cur = nxt;
             }
         }

@@ -598,24 +598,29 @@ static inline double tri_f64_words(const float* t, float qx, float qy, float qz)
     return tri_f64(a, b, c);
 }
 
-static inline float far_field(const float* w, float dx, float dy, float dz, float d2)
+/* omega + inv3 (o0 + inv2 (q1 + inv2 q2)) with q1 and q2 in Horner form: the GPU kernel's
+ * operation order (one rounding per FMA), so the same tree gives the same sum up to rsqrt. */
+static inline float far_field_acc(const float* w, float dx, float dy, float dz, float d2, float om)
 {
-    float inv = 1.0f / sqrtf(d2);
-    float inv2 = inv * inv;
-    float inv3 = inv2 * inv;
-    float inv5 = inv3 * inv2;
-    float inv7 = inv5 * inv2;
-    float o0 = fmaf(dx, w[FWN_W_NX], fmaf(dy, w[FWN_W_NY], dz * w[FWN_W_NZ]));
-    float xx = dx * dx, yy = dy * dy, zz = dz * dz;
-    float xy = dx * dy, yz = dy * dz, zx = dz * dx;
+    const float inv = 1.0f / sqrtf(d2);
+    const float inv2 = inv * inv;
+    const float inv3 = inv2 * inv;
     const float* A = w + FWN_W_A;
-    float q1 = fmaf(A[0], xx, fmaf(A[1], yy, fmaf(A[2], zz, fmaf(A[3], xy, fmaf(A[4], yz, A[5] * zx)))));
-    q1 = fmaf(w[FWN_W_C], dx, fmaf(w[FWN_W_C + 1], dy, fmaf(w[FWN_W_C + 2], dz, q1)));
     const float* Bq = w + FWN_W_B;
-    float q2 = fmaf(Bq[0], xx * dx, fmaf(Bq[1], yy * dy, fmaf(Bq[2], zz * dz,
-               fmaf(Bq[3], xx * dy, fmaf(Bq[4], xx * dz, fmaf(Bq[5], yy * dx,
-               fmaf(Bq[6], yy * dz, fmaf(Bq[7], zz * dx, fmaf(Bq[8], zz * dy, Bq[9] * (xy * dz))))))))));
-    return fmaf(o0 + w[FWN_W_TR], inv3, fmaf(q1, inv5, q2 * inv7));
+    const float* Cq = w + FWN_W_C;
+    const float o0 = fmaf(dx, w[FWN_W_NX], fmaf(dy, w[FWN_W_NY], fmaf(dz, w[FWN_W_NZ], w[FWN_W_TR])));
+    const float a1 = fmaf(A[0], dx, fmaf(A[3], dy, fmaf(A[5], dz, Cq[0])));
+    const float b1 = fmaf(A[1], dy, fmaf(A[4], dz, Cq[1]));
+    const float c1 = fmaf(A[2], dz, Cq[2]);
+    const float q1 = fmaf(dx, a1, fmaf(dy, b1, dz * c1));
+    const float ax = fmaf(Bq[0], dx, fmaf(Bq[3], dy, Bq[4] * dz));
+    const float bx = fmaf(Bq[5], dy, Bq[9] * dz);
+    const float X = fmaf(dx, ax, fmaf(dy, bx, dz * (Bq[7] * dz)));
+    const float Y = fmaf(dy, fmaf(Bq[1], dy, Bq[6] * dz), dz * (Bq[8] * dz));
+    const float Z = dz * (dz * Bq[2]);
+    const float q2 = fmaf(dx, X, fmaf(dy, Y, dz * Z));
+    const float t = fmaf(inv2, fmaf(inv2, q2, q1), o0);
+    return fmaf(t, inv3, om);
 }
 
 typedef struct { int32_t node; int32_t ns; } tstack;
@@ -649,7 +654,7 @@ static float eval_point(const uint32_t* slots, int width, int32_t root, int32_t 
             float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
             nt++;
             if (d2 > beta2 * w[FWN_W_R2]) {
-                om += far_field(w, dx, dy, dz, d2);
+                om = far_field_acc(w, dx, dy, dz, d2, om);
                 nf++;
             } else if (wi[FWN_W_CHILD] < 0) {
                 int32_t first = wi[FWN_W_FIRST], cnt = wi[FWN_W_COUNT];
