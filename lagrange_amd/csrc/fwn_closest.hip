@@ -268,7 +268,7 @@ __global__ __launch_bounds__(64) void k_closest(const uint32_t* __restrict__ slo
             uint32_t e;
             uint64_t mk;
             S.pop(sp, e, mk);
-            const bool act = (mk >> lane) & 1ull;
+            const bool act = __builtin_amdgcn_inverse_ballot_w64(mk); // this lane's bit of mk, as a mask
             const int64_t node = e >> 4;
             const int ns = (int)(e & 15u);
             const uint32_t* nb = slots + (size_t)node * (size_t)width * kSlotWords;

@@ -191,7 +191,7 @@ __device__ __forceinline__ void traverse_batch(
             uint32_t e;
             uint64_t mk;
             S.pop(sp, e, mk);
-            const bool act = (mk >> lane) & 1ull;
+            const bool act = __builtin_amdgcn_inverse_ballot_w64(mk); // this lane's bit of mk, as a mask
             const uint32_t* nb = slots + (size_t)(e >> 4) * (size_t)width * kSlotWords;
             const int ns = (int)(e & 15u);
             if (ns == 0) continue;
@@ -214,7 +214,7 @@ __device__ __forceinline__ void traverse_batch(
                     if (child < 0) {
                         const int32_t first = si(cur, FWN_W_FIRST);
                         const int32_t tcnt = si(cur, FWN_W_COUNT);
-                        const bool nl = (nearm >> lane) & 1ull;
+                        const bool nl = __builtin_amdgcn_inverse_ballot_w64(nearm);
                         // FP64 for points within tau of the leaf's bounding sphere and, for a single
                         // triangle, within tau of its plane (p lies on that plane)
                         const float rr = sqrtf(r2) + tau;
