@@ -104,7 +104,8 @@ def cpu_baseline(V, F, q, sample_stride: int, threads: int):
 
 def bench_batched(steps: int = 5):
     """64 meshes (displaced spheres / tori, 1k..500k triangles, 10k..2M points each, uniform in
-    bbox x1.3): one build and one query launch for all meshes, launch shape from the autotuner."""
+    bbox x1.3): one build and one query launch for all meshes, launch shape from the autotuner;
+    leaf buckets of 2 triangles (the batch of mostly small meshes is fastest there)."""
     import numpy as np
     import torch
 
@@ -113,7 +114,7 @@ def bench_batched(steps: int = 5):
 
     meshes, pts = W.batched_workload()
     t0 = time.perf_counter()
-    mm = MultiFastWindingNumber([(m.vertices, m.facets) for m in meshes])
+    mm = MultiFastWindingNumber([(m.vertices, m.facets) for m in meshes], leaf_max=2)
     build_wall = time.perf_counter() - t0
     q_off = np.zeros(len(pts) + 1, np.int64)
     q_off[1:] = np.cumsum([p.shape[0] for p in pts])
@@ -143,6 +144,7 @@ def bench_batched(steps: int = 5):
         "queries_per_s": n / dt,
         "ms_per_step": dt * 1e3,
         "launch": {"waves_per_block": waves, "mode": "persistent" if mode == 1 else "grid", "chosen_by": "autotune"},
+        "leaf_max": 2,
         "build_ms": info["build_ms"],
         "build_wall_s": build_wall,
         "max_abs_dw": float(err.max()),
@@ -402,7 +404,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", type=int, default=256)
-    ap.add_argument("--leaf-max", type=int, default=4)
+    ap.add_argument("--leaf-max", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=1)
     ap.add_argument("--extras", default="batched,near,voxel,examples,closest,io,surrogate",

@@ -64,7 +64,7 @@ typedef struct fwn_options {
                                  triangles use the FP64 solid angle when q is within tau of the
                                  leaf's bounding sphere and, for a single triangle, within tau of
                                  its plane */
-    int32_t leaf_max;         /* max triangles per leaf bucket (1..8, default 4; 1 gives every
+    int32_t leaf_max;         /* max triangles per leaf bucket (1..8, default 8; 1 gives every
                                  triangle its own far-field expansion, as the reference's items) */
     int32_t fp64_moment_min;  /* tree nodes with at least this many triangles merge moments in FP64 */
     int32_t device;           /* HIP device ordinal */

@@ -161,7 +161,7 @@ int fwn_options_default(fwn_options* opt)
     std::memset(opt, 0, sizeof(*opt));
     opt->accuracy_scale = 2.0f;
     opt->near_fp64_rel = 1e-4f;
-    opt->leaf_max = 4;
+    opt->leaf_max = 8;
     opt->fp64_moment_min = 256;
     opt->device = 0;
     opt->waves_per_block = 1;

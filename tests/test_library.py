@@ -41,7 +41,7 @@ def test_default_options_match_reference_defaults():
     assert o.accuracy_scale == 2.0  # the reference passes no accuracy argument (default 2)
     assert o.near_fp64_rel == pytest.approx(1e-4)
     assert o.tree_width == 4  # the reference's 4-ary tree
-    assert o.leaf_max == 4 and o.waves_per_block == 1  # measured fastest; same criterion and accuracy
+    assert o.leaf_max == 8 and o.waves_per_block == 1  # measured fastest; same criterion and accuracy
 
 
 def test_errors_without_device_or_with_bad_arguments():
