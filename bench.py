@@ -104,8 +104,8 @@ def cpu_baseline(V, F, q, sample_stride: int, threads: int):
 
 def bench_batched(steps: int = 5):
     """64 meshes (displaced spheres / tori, 1k..500k triangles, 10k..2M points each, uniform in
-    bbox x1.3): one build and one query launch for all meshes, launch shape from the autotuner;
-    leaf buckets of 2 triangles (the batch of mostly small meshes is fastest there)."""
+    bbox x1.3): one build and one query launch for all meshes; leaf size and launch shape picked
+    by the tuner (fwn_create_tuned over leaf sizes 1, 2, 4, 8, each with its launch autotune)."""
     import numpy as np
     import torch
 
@@ -113,12 +113,18 @@ def bench_batched(steps: int = 5):
     from lagrange_amd.winding import MultiFastWindingNumber
 
     meshes, pts = W.batched_workload()
-    t0 = time.perf_counter()
-    mm = MultiFastWindingNumber([(m.vertices, m.facets) for m in meshes], leaf_max=2)
-    build_wall = time.perf_counter() - t0
+    arrs = [(m.vertices, m.facets) for m in meshes]
     q_off = np.zeros(len(pts) + 1, np.int64)
     q_off[1:] = np.cumsum([p.shape[0] for p in pts])
     qd = torch.from_numpy(np.concatenate(pts)).cuda()
+    t0 = time.perf_counter()
+    tuned = MultiFastWindingNumber.tuned(arrs, qd, q_off, (1, 2, 4, 8))
+    tune_wall = time.perf_counter() - t0
+    leaf = tuned.tuning["leaf_max"]
+    tuned.close()
+    t0 = time.perf_counter()
+    mm = MultiFastWindingNumber(arrs, leaf_max=leaf)
+    build_wall = time.perf_counter() - t0
     waves, mode, _ = mm.autotune(qd, q_off)
     mm.solid_angle_batch(points=qd, q_off=q_off)
     torch.cuda.synchronize()
@@ -144,7 +150,9 @@ def bench_batched(steps: int = 5):
         "queries_per_s": n / dt,
         "ms_per_step": dt * 1e3,
         "launch": {"waves_per_block": waves, "mode": "persistent" if mode == 1 else "grid", "chosen_by": "autotune"},
-        "leaf_max": 2,
+        "leaf_max": leaf,
+        "leaf_max_chosen_by": "fwn_create_tuned over 1, 2, 4, 8",
+        "tuning_wall_s": tune_wall,
         "build_ms": info["build_ms"],
         "build_wall_s": build_wall,
         "max_abs_dw": float(err.max()),

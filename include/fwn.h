@@ -229,6 +229,16 @@ int fwn_set_launch(fwn_engine* e, int32_t waves_per_block, int32_t mode);
 int fwn_autotune(fwn_engine* e, const float* Q, const int64_t* q_off, int64_t nQ, int32_t flags,
                  const char* cache_path, int32_t* waves_out, int32_t* mode_out, double* best_ms_out);
 
+/* Build-option tuner (no reference counterpart): builds one engine per leaf size in leaf_sizes
+ * (other options from opt, NULL = defaults), autotunes each one's launch shape on the given
+ * points (as fwn_autotune, q_off NULL with nQ points for a single mesh) and keeps the one whose
+ * query call is fastest; the others are destroyed.  Same mesh layout as fwn_create_multi
+ * (n_meshes = 1, v_off = {0, nV}, f_off = {0, nF} for one mesh). */
+int fwn_create_tuned(fwn_engine** out, int32_t n_meshes, const float* V, const int64_t* v_off, const int32_t* F,
+                     const int64_t* f_off, const fwn_options* opt, const float* Q, const int64_t* q_off, int64_t nQ,
+                     int32_t flags, const int32_t* leaf_sizes, int32_t n_leaf_sizes, int32_t* leaf_out,
+                     double* best_ms_out);
+
 /* Average device duration (ms) of the traversal kernel in the most recent query call and the
  * number of launches it took; measured with HIP events on the engine's stream. */
 int fwn_last_timing(const fwn_engine* e, double* traverse_ms, double* total_ms);

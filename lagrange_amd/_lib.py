@@ -111,6 +111,8 @@ def load():
         "fwn_query_stats": ([VP, VP, VP, I64, VP, VP, I32], I32),
         "fwn_bruteforce_f64": ([VP, VP, I64, VP, I32], I32),
         "fwn_last_timing": ([VP, VP, VP], I32),
+        "fwn_create_tuned": ([C.POINTER(VP), I32, VP, VP, VP, VP, C.POINTER(Options), VP, VP, I64, I32, VP, I32, VP,
+                              VP], I32),
         "fwn_set_launch": ([VP, I32, I32], I32),
         "fwn_autotune": ([VP, VP, VP, I64, I32, C.c_char_p, VP, VP, VP], I32),
         "fwn_last_error": ([], C.c_char_p),

@@ -445,6 +445,50 @@ int fwn_autotune(fwn_engine* h, const float* Q, const int64_t* q_off, int64_t nQ
     });
 }
 
+int fwn_create_tuned(fwn_engine** out, int32_t n_meshes, const float* V, const int64_t* v_off, const int32_t* F,
+                     const int64_t* f_off, const fwn_options* opt, const float* Q, const int64_t* q_off, int64_t nQ,
+                     int32_t flags, const int32_t* leaf_sizes, int32_t n_leaf_sizes, int32_t* leaf_out,
+                     double* best_ms_out)
+{
+    return guarded("fwn_create_tuned", [&] {
+        if (!out) fwn::fail(FWN_INVALID_ARGUMENT, "out is NULL");
+        *out = nullptr;
+        if (!leaf_sizes || n_leaf_sizes < 1) fwn::fail(FWN_INVALID_ARGUMENT, "no leaf sizes to try");
+        fwn_options o;
+        if (opt) o = *opt;
+        else fwn_options_default(&o);
+        fwn_engine* best = nullptr;
+        double best_ms = 1e300;
+        int32_t best_leaf = -1;
+        for (int32_t k = 0; k < n_leaf_sizes; ++k) {
+            o.leaf_max = leaf_sizes[k];
+            fwn_engine* e = nullptr;
+            int st = fwn_create_multi(&e, n_meshes, V, v_off, F, f_off, &o);
+            if (st == FWN_OK) {
+                int32_t w = 0, m = 0;
+                double ms = 0.0;
+                st = fwn_autotune(e, Q, q_off, nQ, flags, nullptr, &w, &m, &ms);
+                if (st == FWN_OK && ms < best_ms) {
+                    if (best) fwn_destroy(best);
+                    best = e;
+                    best_ms = ms;
+                    best_leaf = o.leaf_max;
+                    continue;
+                }
+                fwn_destroy(e);
+            }
+            if (st != FWN_OK) {
+                const std::string msg = fwn::last_error_text();
+                if (best) fwn_destroy(best);
+                fwn::fail(st, msg);
+            }
+        }
+        *out = best;
+        if (leaf_out) *leaf_out = best_leaf;
+        if (best_ms_out) *best_ms_out = best_ms;
+    });
+}
+
 int fwn_last_timing(const fwn_engine* h, double* traverse_ms, double* total_ms)
 {
     return guarded("fwn_last_timing", [&] {

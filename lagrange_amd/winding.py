@@ -49,6 +49,26 @@ def _ptr(a) -> int:
     return a.ctypes.data if isinstance(a, np.ndarray) else int(a.data_ptr())
 
 
+def _create_tuned(n_meshes, V, v_off, F, f_off, points, q_off, leaf_sizes, options, opts):
+    """Shared body of the tuned constructors: (handle, chosen leaf size, query ms)."""
+    o = options if options is not None else L.default_options(**opts)
+    leaves = np.ascontiguousarray(np.asarray(leaf_sizes, dtype=np.int32).reshape(-1))
+    qo = None if q_off is None else np.ascontiguousarray(q_off, dtype=np.int64)
+    if _is_torch(points) and points.is_cuda:
+        import torch
+        q = points.contiguous().to(torch.float32)
+        torch.cuda.synchronize(q.device)
+        n, ptr, fl = q.shape[0], _ptr(q), L.FWN_DEVICE_POINTERS
+    else:
+        q = np.ascontiguousarray(np.asarray(points, dtype=np.float32).reshape(-1, 3))
+        n, ptr, fl = q.shape[0], q.ctypes.data, 0
+    h, leaf, ms = C.c_void_p(), C.c_int32(), C.c_double()
+    L.check(L.load().fwn_create_tuned(C.byref(h), n_meshes, V.ctypes.data, v_off.ctypes.data, F.ctypes.data,
+                                      f_off.ctypes.data, C.byref(o), ptr, qo.ctypes.data if qo is not None else None,
+                                      n, fl, leaves.ctypes.data, leaves.size, C.byref(leaf), C.byref(ms)))
+    return h, leaf.value, ms.value
+
+
 class _EngineBase:
     _handle = None
     _dependents = None
@@ -143,6 +163,23 @@ class FastWindingNumber(_EngineBase):
         # float bounding box of the mesh (mesh_bbox<3> of a float mesh), for sampling
         self.bbox = ((V.min(axis=0), V.max(axis=0)) if V.shape[0] else
                      (np.zeros(3, np.float32), np.zeros(3, np.float32)))
+
+    @classmethod
+    def tuned(cls, mesh, points, leaf_sizes=(1, 2, 4, 8), *, options: L.Options | None = None, **opts):
+        """Engine built with the leaf size (and launch shape) that answers ``points`` fastest
+        (fwn_create_tuned); ``engine.tuning`` holds the choice and its query time."""
+        V, F = _mesh_arrays(mesh)
+        v_off = np.array([0, V.shape[0]], np.int64)
+        f_off = np.array([0, F.shape[0]], np.int64)
+        h, leaf, ms = _create_tuned(1, V, v_off, F, f_off, points, None, leaf_sizes, options, opts)
+        self = cls.__new__(cls)
+        self._handle = h
+        self.num_vertices = V.shape[0]
+        self.num_facets = F.shape[0]
+        self.bbox = ((V.min(axis=0), V.max(axis=0)) if V.shape[0] else
+                     (np.zeros(3, np.float32), np.zeros(3, np.float32)))
+        self.tuning = {"leaf_max": leaf, "query_ms": ms}
+        return self
 
     # ---- per-point API (as the reference) ------------------------------------------------
     def solid_angle(self, pos) -> float:
@@ -343,6 +380,27 @@ class MultiFastWindingNumber(_EngineBase):
                                           F.ctypes.data, self.f_off.ctypes.data, C.byref(o)))
         self._handle = h
         self.num_meshes = len(arrs)
+
+    @classmethod
+    def tuned(cls, meshes, points, q_off, leaf_sizes=(1, 2, 4, 8), *, options: L.Options | None = None, **opts):
+        """Engine built with the leaf size (and launch shape) that answers the concatenated
+        ``points`` (offsets ``q_off``) fastest; ``engine.tuning`` holds the choice."""
+        arrs = [_mesh_arrays(m) for m in meshes]
+        if not arrs:
+            raise LagrangeError("at least one mesh is required")
+        V = np.ascontiguousarray(np.concatenate([a[0] for a in arrs]), dtype=np.float32)
+        F = np.ascontiguousarray(np.concatenate([a[1] for a in arrs]), dtype=np.int32)
+        v_off = np.zeros(len(arrs) + 1, np.int64)
+        f_off = np.zeros(len(arrs) + 1, np.int64)
+        v_off[1:] = np.cumsum([a[0].shape[0] for a in arrs])
+        f_off[1:] = np.cumsum([a[1].shape[0] for a in arrs])
+        h, leaf, ms = _create_tuned(len(arrs), V, v_off, F, f_off, points, q_off, leaf_sizes, options, opts)
+        self = cls.__new__(cls)
+        self._handle = h
+        self.v_off, self.f_off = v_off, f_off
+        self.num_meshes = len(arrs)
+        self.tuning = {"leaf_max": leaf, "query_ms": ms}
+        return self
 
     def solid_angle_batch(self, points_per_mesh=None, *, points=None, q_off=None, with_inside=False):
         """Either a list of (Ni, 3) arrays (one per mesh) or one concatenated array + offsets."""

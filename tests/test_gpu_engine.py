@@ -320,3 +320,24 @@ def test_autotune_picks_valid_shape_and_caches(tmp_path):
     e2 = FastWindingNumber((V, F))
     assert e2.autotune(q, cache_path=cache)[:2] == (w, mode)  # read back from the cache
     assert open(cache).read().count("\n") == 1
+
+
+def test_build_tuner_keeps_the_fastest_leaf_size():
+    """fwn_create_tuned returns an engine built with one of the candidate leaf sizes whose results
+    equal an engine built directly with that leaf size; bad candidates are rejected."""
+    meshes = [W.displaced_sphere(14, 3, delete_fraction=0.05), W.torus(40, 16, seed=2, amplitude=0.2)]
+    pts = [W.uniform_queries(*m.bbox(), n, 80 + i, 1.3) for i, (m, n) in enumerate(zip(meshes, [20000, 9000]))]
+    q_off = np.array([0, pts[0].shape[0], pts[0].shape[0] + pts[1].shape[0]], np.int64)
+    allq = np.concatenate(pts)
+    arrs = [(m.vertices, m.facets) for m in meshes]
+    mm = MultiFastWindingNumber.tuned(arrs, allq, q_off, (1, 4, 8))
+    assert mm.tuning["leaf_max"] in (1, 4, 8) and mm.tuning["query_ms"] > 0
+    direct = MultiFastWindingNumber(arrs, leaf_max=mm.tuning["leaf_max"])
+    np.testing.assert_array_equal(mm.solid_angle_batch(points=allq, q_off=q_off),
+                                  direct.solid_angle_batch(points=allq, q_off=q_off))
+    single = FastWindingNumber.tuned((meshes[0].vertices, meshes[0].facets), pts[0], (2, 8))
+    assert single.tuning["leaf_max"] in (2, 8)
+    ref = FastWindingNumber((meshes[0].vertices, meshes[0].facets), leaf_max=single.tuning["leaf_max"])
+    np.testing.assert_array_equal(single.solid_angle_batch(pts[0]), ref.solid_angle_batch(pts[0]))
+    with pytest.raises(L.FWNError):
+        FastWindingNumber.tuned((meshes[0].vertices, meshes[0].facets), pts[0], (3, 9))
