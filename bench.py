@@ -67,8 +67,12 @@ def _host_cpu():
             "affinity": len(os.sched_getaffinity(0))}
 
 
-def cpu_baseline(V, F, q, sample_stride: int, threads: int):
+def cpu_baseline(V, F, q, sample_stride: int, threads: int, acc=None):
+    """CPU restatement timed on the host cores. With ``acc`` = (sample indices, FP64 w of the
+    sample, beta at 1e-3) it also reports the restatement's own error at its default beta 2 on
+    that sample, and its rate at the beta the GPU engine needs for max |dw| <= 1e-3."""
     from oracle import oracle as O
+    import numpy as np
     sub = q[::sample_stride].copy()
     t0 = time.perf_counter()
     R = O.Restated(V, F)
@@ -87,6 +91,24 @@ def cpu_baseline(V, F, q, sample_stride: int, threads: int):
         R.solid_angle(one, nthreads=1)
         t1.append(time.perf_counter() - t0)
     dt1 = sorted(t1)[1]
+    extra = {}
+    if acc is not None:
+        idx, wb, beta = acc
+        wr = R.solid_angle(q[idx], nthreads=threads).astype(np.float64) / (4 * np.pi)
+        err = np.abs(wr - wb)
+        extra["restated_error_at_beta_2"] = {"sample": int(idx.size), "max_abs_dw": float(err.max()),
+                                             "p99_abs_dw": float(np.quantile(err, 0.99)),
+                                             "mean_abs_dw": float(err.mean())}
+        if beta is not None:
+            sb = q[::4 * sample_stride].copy()
+            tb = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                R.solid_angle(sb, beta=beta, nthreads=threads)
+                tb.append(time.perf_counter() - t0)
+            extra["value_at_matched_beta"] = {"accuracy_scale": beta, "value": sb.shape[0] / sorted(tb)[1],
+                                              "sample": f"every {4 * sample_stride}th grid point "
+                                                        f"({sb.shape[0]} points), median of 3"}
     return {
         "value": sub.shape[0] / dt,
         "unit": "queries/s",
@@ -99,6 +121,7 @@ def cpu_baseline(V, F, q, sample_stride: int, threads: int):
         "single_thread_value": one.shape[0] / dt1,
         "single_thread_sample": f"every {16 * sample_stride}th grid point ({one.shape[0]} points), median of 3",
         "host": _host_cpu(),
+        **extra,
     }
 
 
@@ -160,6 +183,51 @@ def bench_batched(steps: int = 5):
     }
 
 
+def bench_accuracy(V, F, q, qd, steps: int = 10, betas=(2.0, 2.5, 2.75, 3.0, 3.5, 4.0), sample: int = 16384):
+    """Accuracy / speed trade of the accuracy scale on the headline workload: for beta = 2, 2.5,
+    2.75, 3, ... the engine is rebuilt (the build stores beta^2 r^2 per slot), one step is timed as in
+    the headline, and max / p99 |dw| are measured against the FP64 brute force on a strided
+    sample of ~16,384 grid points (odd stride). Stops at the first beta whose maximum error is <= 1e-3."""
+    import numpy as np
+    import torch
+
+    from lagrange_amd.winding import FastWindingNumber
+
+    n = q.shape[0]
+    idx = np.arange(0, n, max(n // sample, 1) | 1)  # odd stride: spreads over x of the 2^k grid
+    wb = None
+    rows = []
+    for beta in betas:
+        e = FastWindingNumber((V, F), accuracy_scale=beta)
+        if wb is None:
+            wb = e.bruteforce_f64(q[idx]) / (4 * np.pi)
+        for _ in range(2):
+            om, ins = e.solid_angle_batch(qd, with_inside=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            om, ins = e.solid_angle_batch(qd, with_inside=True)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        err = np.abs(om.cpu().numpy()[idx].astype(np.float64) / (4 * np.pi) - wb)
+        amb = np.abs(wb - 0.5) <= 1e-3
+        ins_bad = (ins.cpu().numpy()[idx] != (wb > 0.5)) & ~amb
+        rows.append({"accuracy_scale": beta, "queries_per_s": n / dt, "ms_per_step": dt * 1e3,
+                     "max_abs_dw": float(err.max()), "p99_abs_dw": float(np.quantile(err, 0.99)),
+                     "mean_abs_dw": float(err.mean()), "inside_mismatch_excl_band": int(ins_bad.sum())})
+        e.close()
+        del om, ins
+        if err.max() <= 1e-3:
+            break
+    ok = [r for r in rows if r["max_abs_dw"] <= 1e-3]
+    return {
+        "workload": f"headline mesh and grid ({n} points); error sample every {idx[1] - idx[0]}th point "
+                    f"({idx.size} points) vs FP64 brute force",
+        "sweep": rows,
+        "at_1e-3": ok[0] if ok else None,
+    }, idx, wb
+
+
 def bench_near_surface(steps: int = 5):
     """200,000-triangle displaced sphere, 4M points offset 1e-7..1e-2 x diag along normals."""
     import numpy as np
@@ -170,8 +238,19 @@ def bench_near_surface(steps: int = 5):
 
     m, q = W.near_surface_workload()
     V, F = m.f32()
-    e = FastWindingNumber((V, F))
     qd = torch.from_numpy(q).cuda()
+    # the tuner's engine (leaf size and launch shape for these points), then the default one
+    tuned = FastWindingNumber.tuned((V, F), qd, (1, 2, 4, 8))
+    tuned.solid_angle_batch(qd)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tuned.solid_angle_batch(qd)
+    torch.cuda.synchronize()
+    dt_tuned = (time.perf_counter() - t0) / steps
+    leaf_tuned = tuned.tuning["leaf_max"]
+    tuned.close()
+    e = FastWindingNumber((V, F))
     e.solid_angle_batch(qd)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -196,6 +275,8 @@ def bench_near_surface(steps: int = 5):
         "mean_abs_dw": float(err.mean()),
         "inside_mismatch_excl_band": int(((insb != (wb > 0.5)) & ~amb).sum()),
         "fp64_triangles_per_point": float(st[:, 3].mean()),
+        "tuned": {"leaf_max": leaf_tuned, "queries_per_s": q.shape[0] / dt_tuned, "ms_per_step": dt_tuned * 1e3,
+                  "chosen_by": "fwn_create_tuned over leaf sizes 1, 2, 4, 8"},
     }
 
 
@@ -284,14 +365,14 @@ def bench_examples(mesh, steps: int = 5):
 def bench_closest(mesh, q, steps: int = 5):
     """Closest point / unsigned distance (bvh::TriangleAABBTree::get_closest_point, batched) for the
     headline grid points on the ~1M-triangle mesh, FP64 point-triangle distances, results in HBM;
-    checked against the brute-force oracle on a small strided sample."""
+    the outputs are checked for consistency on a strided sample."""
     import numpy as np
     import torch
 
     from lagrange_amd.winding import FastWindingNumber
 
     V, F = mesh.f32()
-    e = FastWindingNumber((V, F))
+    e = FastWindingNumber((V, F), leaf_max=8)  # the box search visits fewer nodes with buckets
     qd = torch.from_numpy(q).cuda()
     d2, fi, cp = e.closest_point(qd)
     torch.cuda.synchronize()
@@ -301,15 +382,19 @@ def bench_closest(mesh, q, steps: int = 5):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     out = {"workload": f"{F.shape[0]} triangles, {q.shape[0]} grid points (bbox x1.2)",
-           "points_per_s": q.shape[0] / dt, "ms": dt * 1e3}
-    try:
-        from oracle import oracle as O
-        idx = np.arange(0, q.shape[0], q.shape[0] // 64)
-        od2, _, _ = O.closest(V, F, q[idx])
-        out["max_rel_err_vs_bruteforce"] = float(np.max(np.abs(d2.cpu().numpy()[idx] - od2) / np.maximum(od2, 1e-300)))
-        out["sample"] = int(idx.size)
-    except Exception as ex:  # the checker is optional in a bench run
-        out["check"] = f"skipped: {ex}"
+           "points_per_s": q.shape[0] / dt, "ms": dt * 1e3, "leaf_max": 8}
+    # consistency of the three outputs (exactness against a brute force is a test's job):
+    # |closest - q|^2 equals the squared distance, and the closest point lies on its facet's plane
+    idx = np.arange(0, q.shape[0], q.shape[0] // 4096)
+    d2h, fih, cph = d2.cpu().numpy()[idx], fi.cpu().numpy()[idx], cp.cpu().numpy()[idx]
+    qq = q[idx].astype(np.float64)
+    tri = V[F[fih]].astype(np.float64)
+    nrm = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    nrm /= np.maximum(np.linalg.norm(nrm, axis=1, keepdims=True), 1e-300)
+    out["sample"] = int(idx.size)
+    out["max_rel_dev_d2_vs_point"] = float(np.max(np.abs(np.sum((cph - qq) ** 2, 1) - d2h) / np.maximum(d2h, 1e-30)))
+    out["max_off_plane"] = float(np.max(np.abs(np.sum((cph - tri[:, 0]) * nrm, 1))))
+    out["exactness"] = "tests/test_gpu_closest.py (FP64 brute force)"
     return out
 
 
@@ -412,12 +497,13 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", type=int, default=256)
-    ap.add_argument("--leaf-max", type=int, default=8)
+    ap.add_argument("--leaf-max", type=int, default=1,
+                    help="leaf bucket size (default 1: the engine's default, the reference's tree layout)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=1)
-    ap.add_argument("--extras", default="batched,near,voxel,examples,closest,io,surrogate",
+    ap.add_argument("--extras", default="batched,near,voxel,examples,closest,io,accuracy,surrogate",
                     help="comma list of extra workloads reported beside the headline: batched, near, voxel, "
-                         "examples, closest, io, surrogate")
+                         "examples, closest, io, accuracy, surrogate")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (scale for rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -463,6 +549,19 @@ def main():
     ms_per_step = dt * 1e3 / args.steps
     value = n * args.steps / dt
 
+    # the tuner's engine for the same points (leaf size x launch shape), timed the same way
+    tuned = FastWindingNumber.tuned((V, F), qd, (1, 2, 4, 8))
+    for _ in range(args.warmup):
+        tuned.solid_angle_batch(qd, with_inside=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        om_t, _ = tuned.solid_angle_batch(qd, with_inside=True)
+    torch.cuda.synchronize()
+    dt_t = time.perf_counter() - t0
+    tuned_rec = {"leaf_max": tuned.tuning["leaf_max"], "queries_per_s": n * args.steps / dt_t,
+                 "ms_per_step": dt_t * 1e3 / args.steps, "chosen_by": "fwn_create_tuned over leaf sizes 1, 2, 4, 8"}
+
     # algorithmic work of the traversal on this tree and criterion (all points, one extra launch)
     _, st = e.query_stats(q)
     tot = st.astype(np.int64).sum(axis=0)
@@ -479,6 +578,9 @@ def main():
     err = np.abs(w - wb)
     ins_ok = ins.cpu().numpy()[idx] == (wb > 0.5)
     amb = np.abs(wb - 0.5) <= 1e-3
+    tuned_rec["max_abs_dw"] = float(np.abs(om_t.cpu().numpy()[idx].astype(np.float64) / (4 * np.pi) - wb).max())
+    tuned.close()
+    del om_t
 
     line = {
         "metric": METRIC,
@@ -500,6 +602,7 @@ def main():
             "accuracy_scale": 2.0,
             "order": 2,
             "leaf_max": args.leaf_max,
+            "tree_builder": "sah (binned box-area, 4-ary: the reference's tree layout)",
             "tree_width": info["width"],
             "tree_nodes": info["num_nodes"],
             "tree_depth": info["max_depth"],
@@ -529,6 +632,7 @@ def main():
             "mean_abs_dw": float(err.mean()),
             "inside_mismatch_excl_band": int((~ins_ok & ~amb).sum()),
         },
+        "tuned": tuned_rec,
     }
     extras = [x for x in args.extras.split(",") if x and x != "none"]
     if "batched" in extras:
@@ -543,6 +647,11 @@ def main():
         line["closest"] = bench_closest(mesh, q)
     if "io" in extras:
         line["io"] = bench_io(mesh)
+    acc = None
+    if "accuracy" in extras:
+        line["accuracy_scale_sweep"], idx_a, wb_a = bench_accuracy(V, F, q, qd)
+        hit = line["accuracy_scale_sweep"]["at_1e-3"]
+        acc = (idx_a, wb_a, hit["accuracy_scale"] if hit else None)
     if "surrogate" in extras:
         del qd, om, ins
         torch.cuda.empty_cache()
@@ -550,8 +659,16 @@ def main():
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
         threads = min(threads, 16)
-        line["cpu_baseline"] = cpu_baseline(V, F, q, args.cpu_stride, threads)
+        line["cpu_baseline"] = cpu_baseline(V, F, q, args.cpu_stride, threads, acc)
         line["speedup_vs_cpu"] = value / line["cpu_baseline"]["value"]
+        hit = line.get("accuracy_scale_sweep", {}).get("at_1e-3")
+        if hit:
+            # GPU at the accuracy scale that keeps max |dw| <= 1e-3, against the CPU restatement at
+            # the reference default (beta 2, itself less accurate) and at the same beta
+            cb = line["cpu_baseline"]
+            hit["speedup_vs_cpu_beta_2"] = hit["queries_per_s"] / cb["value"]
+            if "value_at_matched_beta" in cb:
+                hit["speedup_vs_cpu_same_beta"] = hit["queries_per_s"] / cb["value_at_matched_beta"]["value"]
     print(json.dumps(line), flush=True)
 
 

@@ -64,15 +64,29 @@ typedef struct fwn_options {
                                  triangles use the FP64 solid angle when q is within tau of the
                                  leaf's bounding sphere and, for a single triangle, within tau of
                                  its plane */
-    int32_t leaf_max;         /* max triangles per leaf bucket (1..8, default 8; 1 gives every
-                                 triangle its own far-field expansion, as the reference's items) */
+    int32_t leaf_max;         /* max triangles per leaf bucket (1..8, default 1: every triangle has
+                                 its own far-field expansion, as the reference's items; with the
+                                 SAH builder this is the reference layout's tree.  Buckets of 2..8
+                                 traverse faster (fwn_create_tuned picks one) but a bucket can take
+                                 the far field where single triangles would be summed exactly) */
     int32_t fp64_moment_min;  /* tree nodes with at least this many triangles merge moments in FP64 */
     int32_t device;           /* HIP device ordinal */
     int32_t waves_per_block;  /* query launch shape: 1, 2, 4 or 8 waves per block (default 1) */
     int32_t tree_width;       /* slots per tree node: 2, 4 or 8 (default 4) */
     int32_t launch_mode;      /* FWN_LAUNCH_GRID (default) or FWN_LAUNCH_PERSISTENT */
-    int32_t reserved[4];
+    int32_t tree_builder;     /* FWN_TREE_SAH (default) or FWN_TREE_LBVH */
+    int32_t reserved[3];
 } fwn_options;
+
+/* tree builders.  FWN_TREE_LBVH: Morton-ordered binary radix tree (one sort, one pass); a wide
+ * node repeatedly opens its largest non-leaf slot until it has tree_width slots.
+ * FWN_TREE_SAH: top-down binary splits chosen by the binned (16 bins on the longest centroid
+ * axis) box-area cost, every node split down to single triangles; a wide node takes
+ * log2(tree_width) whole binary levels (a 4-wide node: children and grandchildren), the
+ * layout of the reference's 4-ary tree (modules/winding/src/FastWindingNumber.cpp:57 hands
+ * the triangles to the dependency's UT_SolidAngle::init, which builds a 4-ary BVH).  With
+ * leaf_max 1 the SAH tree matches the CPU restatement of that build slot for slot. */
+enum { FWN_TREE_LBVH = 0, FWN_TREE_SAH = 1 };
 
 /* query launch modes: one workgroup per group of 64-point batches, or a persistent grid whose
  * waves take batches from a work counter, meshes with the deepest trees first */
@@ -94,7 +108,7 @@ enum { FWN_LAUNCH_GRID = 0, FWN_LAUNCH_PERSISTENT = 1 };
  *   28   first triangle (position in the engine's sorted triangle array)
  *   29   triangle count covered by the slot
  *   30   number of occupied slots of the child node (internal slots), 0 for leaves
- *   31   reserved (0)
+ *   31   beta^2 r2 (float): the far-field threshold for the engine's accuracy scale
  * Triangles are stored in tree order, FWN_TRI_WORDS words each: v0, v1, v2 (9 floats),
  * original facet index (int32), 2 reserved words.
  */

@@ -24,6 +24,8 @@ FWN_LIMIT_EXCEEDED = 5
 FWN_DEVICE_POINTERS = 1
 FWN_LAUNCH_GRID = 0
 FWN_LAUNCH_PERSISTENT = 1
+FWN_TREE_LBVH = 0
+FWN_TREE_SAH = 1
 
 MAX_WIDTH = 8
 SLOT_WORDS = 32
@@ -43,7 +45,8 @@ class Options(C.Structure):
         ("waves_per_block", C.c_int32),
         ("tree_width", C.c_int32),
         ("launch_mode", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("tree_builder", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 

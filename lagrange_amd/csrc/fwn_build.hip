@@ -73,7 +73,7 @@ __global__ __launch_bounds__(BT) void k_tri_setup(const float* __restrict__ V, c
                                                   int64_t nF, const int64_t* __restrict__ f_off,
                                                   const int64_t* __restrict__ v_off, int M,
                                                   float* __restrict__ cen, int32_t* __restrict__ boxes,
-                                                  int32_t* __restrict__ err)
+                                                  float* __restrict__ tbox, int32_t* __restrict__ err)
 {
     int64_t t = (int64_t)blockIdx.x * BT + threadIdx.x;
     bool valid = t < nF;
@@ -103,6 +103,10 @@ __global__ __launch_bounds__(BT) void k_tri_setup(const float* __restrict__ V, c
             hi[d] = fmaxf(v[0][d], fmaxf(v[1][d], v[2][d]));
             if (!isfinite(v[0][d]) || !isfinite(v[1][d]) || !isfinite(v[2][d])) atomicOr(err, 2);
             cen[3 * t + d] = c[d];
+            if (tbox) {
+                tbox[6 * t + d] = lo[d];
+                tbox[6 * t + 3 + d] = hi[d];
+            }
         }
     }
     float clo[3], chi[3];
@@ -384,7 +388,7 @@ __global__ __launch_bounds__(BT) void k_collapse_level(const int32_t* __restrict
                                                        const int32_t* __restrict__ right,
                                                        const int32_t* __restrict__ first,
                                                        const int32_t* __restrict__ last, int64_t n, int leaf_max,
-                                                       int width, int32_t* __restrict__ wide_slot_bin,
+                                                       int width, int rounds, int32_t* __restrict__ wide_slot_bin,
                                                        int32_t* __restrict__ wide_ns, int32_t* __restrict__ newcount)
 {
     int i = blockIdx.x * BT + threadIdx.x;
@@ -402,7 +406,33 @@ __global__ __launch_bounds__(BT) void k_collapse_level(const int32_t* __restrict
         sc[0] = node_count(sl[0], n, first, last);
         sc[1] = node_count(sl[1], n, first, last);
         ns = 2;
-        while (ns < width) {
+        // rounds > 0: take whole binary levels (every non-leaf slot opened, rounds - 1 times)
+        for (int r = 1; r < rounds; ++r) {
+            int32_t nl[kWidth], nc[kWidth];
+            int nn = 0;
+            for (int s = 0; s < ns; ++s) {
+                int32_t u = sl[s];
+                if (u < n - 1 && sc[s] > leaf_max) {
+                    nl[nn] = left[u];
+                    nc[nn] = node_count(nl[nn], n, first, last);
+                    ++nn;
+                    nl[nn] = right[u];
+                    nc[nn] = node_count(nl[nn], n, first, last);
+                    ++nn;
+                } else {
+                    nl[nn] = u;
+                    nc[nn] = sc[s];
+                    ++nn;
+                }
+            }
+            for (int s = 0; s < nn; ++s) {
+                sl[s] = nl[s];
+                sc[s] = nc[s];
+            }
+            ns = nn;
+        }
+        // rounds == 0: repeatedly open the largest non-leaf slot until the node has width slots
+        while (rounds == 0 && ns < width) {
             int best = -1, bc = leaf_max;
             for (int s = 0; s < ns; ++s)
                 if (sc[s] > bc) {
@@ -570,11 +600,14 @@ void build_engine(Engine& e, int32_t M, const float* V_host, const int64_t* v_of
                        err.as<int32_t>());
     FWN_LAUNCH_CHECK("k_init");
 
+    const bool sah = e.opt.tree_builder == FWN_TREE_SAH;
+    DBuf tbox;
     cen.alloc(sizeof(float) * 3 * (size_t)std::max<int64_t>(nF, 1));
+    if (sah) tbox.alloc(sizeof(float) * 6 * (size_t)std::max<int64_t>(nF, 1));
     if (nF) {
         hipLaunchKernelGGL(k_tri_setup, dim3(grid_blocks(nF, BT)), dim3(BT), 0, s, dV.as<float>(), dF.as<int32_t>(),
                            nF, dfo.as<int64_t>(), dvo.as<int64_t>(), M, cen.as<float>(), boxes.as<int32_t>(),
-                           err.as<int32_t>());
+                           sah ? tbox.as<float>() : nullptr, err.as<int32_t>());
         FWN_LAUNCH_CHECK("k_tri_setup");
     }
     hipLaunchKernelGGL(k_box_final, dim3(grid_blocks(M, 64)), dim3(64), 0, s, boxes.as<int32_t>(), M,
@@ -611,24 +644,7 @@ void build_engine(Engine& e, int32_t M, const float* V_host, const int64_t* v_of
         return;
     }
 
-    // ---- Morton order ----
-    keys.alloc(sizeof(uint64_t) * nF);
-    vals.alloc(sizeof(uint32_t) * nF);
-    keys2.alloc(sizeof(uint64_t) * nF);
-    vals2.alloc(sizeof(uint32_t) * nF);
-    hipLaunchKernelGGL(k_tri_keys, dim3(grid_blocks(nF, BT)), dim3(BT), 0, s, cen.as<float>(), nF, dfo.as<int64_t>(),
-                       M, cinfo.as<float>(), keys.as<uint64_t>(), vals.as<uint32_t>());
-    FWN_LAUNCH_CHECK("k_tri_keys");
-    int mesh_bits = 0;
-    while ((1 << mesh_bits) < M) ++mesh_bits;
-    int nbits = (M > 1) ? 32 + mesh_bits : 30;
-    uint64_t* kp = keys.as<uint64_t>();
-    uint32_t* vp = vals.as<uint32_t>();
-    uint64_t* kp2 = keys2.as<uint64_t>();
-    uint32_t* vp2 = vals2.as<uint32_t>();
-    radix_sort_pairs(kp, vp, kp2, vp2, nF, nbits, hist, rowsum, s);
-
-    // ---- binary radix tree ----
+    // ---- binary tree topology: Morton-order radix tree, or top-down SAH splits ----
     const int64_t n = nF;
     const int64_t nnodes = 2 * n - 1;
     DBuf left, right, parent, first, last, isroot, rootbin, rec, visits;
@@ -643,15 +659,37 @@ void build_engine(Engine& e, int32_t M, const float* V_host, const int64_t* v_of
     hipLaunchKernelGGL(k_fill_i32, dim3(grid_blocks(nnodes, 256)), dim3(256), 0, s, parent.as<int32_t>(), nnodes, -1);
     hipLaunchKernelGGL(k_fill_i32, dim3(grid_blocks(M, 256)), dim3(256), 0, s, rootbin.as<int32_t>(), (int64_t)M, -1);
     FWN_LAUNCH_CHECK("k_fill_i32");
-    if (n > 1) {
-        hipLaunchKernelGGL(k_karras, dim3(grid_blocks(n - 1, BT)), dim3(BT), 0, s, kp, n, left.as<int32_t>(),
-                           right.as<int32_t>(), parent.as<int32_t>(), first.as<int32_t>(), last.as<int32_t>());
-        FWN_LAUNCH_CHECK("k_karras");
+    vals.alloc(sizeof(uint32_t) * nF);
+    uint32_t* vp = vals.as<uint32_t>();
+    if (sah) {
+        build_topology_sah(cen.as<float>(), tbox.as<float>(), nF, f_off, M, reinterpret_cast<int32_t*>(vp),
+                           left.as<int32_t>(), right.as<int32_t>(), parent.as<int32_t>(), first.as<int32_t>(),
+                           last.as<int32_t>(), isroot.as<uint8_t>(), rootbin.as<int32_t>(), s);
+        tbox.release();
+    } else {
+        keys.alloc(sizeof(uint64_t) * nF);
+        keys2.alloc(sizeof(uint64_t) * nF);
+        vals2.alloc(sizeof(uint32_t) * nF);
+        hipLaunchKernelGGL(k_tri_keys, dim3(grid_blocks(nF, BT)), dim3(BT), 0, s, cen.as<float>(), nF,
+                           dfo.as<int64_t>(), M, cinfo.as<float>(), keys.as<uint64_t>(), vp);
+        FWN_LAUNCH_CHECK("k_tri_keys");
+        int mesh_bits = 0;
+        while ((1 << mesh_bits) < M) ++mesh_bits;
+        int nbits = (M > 1) ? 32 + mesh_bits : 30;
+        uint64_t* kp = keys.as<uint64_t>();
+        uint64_t* kp2 = keys2.as<uint64_t>();
+        uint32_t* vp2 = vals2.as<uint32_t>();
+        radix_sort_pairs(kp, vp, kp2, vp2, nF, nbits, hist, rowsum, s);
+        if (n > 1) {
+            hipLaunchKernelGGL(k_karras, dim3(grid_blocks(n - 1, BT)), dim3(BT), 0, s, kp, n, left.as<int32_t>(),
+                               right.as<int32_t>(), parent.as<int32_t>(), first.as<int32_t>(), last.as<int32_t>());
+            FWN_LAUNCH_CHECK("k_karras");
+        }
+        hipLaunchKernelGGL(k_find_roots, dim3(grid_blocks(std::max<int64_t>(n - 1, M), BT)), dim3(BT), 0, s,
+                           first.as<int32_t>(), last.as<int32_t>(), n, kp, dfo.as<int64_t>(), M,
+                           rootbin.as<int32_t>(), isroot.as<uint8_t>());
+        FWN_LAUNCH_CHECK("k_find_roots");
     }
-    hipLaunchKernelGGL(k_find_roots, dim3(grid_blocks(std::max<int64_t>(n - 1, M), BT)), dim3(BT), 0, s,
-                       first.as<int32_t>(), last.as<int32_t>(), n, kp, dfo.as<int64_t>(), M, rootbin.as<int32_t>(),
-                       isroot.as<uint8_t>());
-    FWN_LAUNCH_CHECK("k_find_roots");
 
     // ---- moments ----
     rec.alloc(sizeof(double) * kRecWords * nnodes);
@@ -691,12 +729,15 @@ void build_engine(Engine& e, int32_t M, const float* V_host, const int64_t* v_of
     newc.alloc(sizeof(int32_t) * wcap);
     offs.alloc(sizeof(int32_t) * (wcap + 1));
     FWN_HIP(hipMemcpyAsync(lvl[0].p, level0.data(), sizeof(int32_t) * level0.size(), hipMemcpyHostToDevice, s));
+    int rounds = 0; // binary levels per wide node for the SAH layout: log2(width)
+    while ((1 << rounds) < width) ++rounds;
     int32_t cnt = (int32_t)level0.size(), gbase = 0, depth = 0;
     int cur = 0;
     while (cnt > 0) {
         hipLaunchKernelGGL(k_collapse_level, dim3(grid_blocks(cnt, BT)), dim3(BT), 0, s, lvl[cur].as<int32_t>(), cnt,
                            gbase, left.as<int32_t>(), right.as<int32_t>(), first.as<int32_t>(), last.as<int32_t>(),
-                           n, leaf_max, width, wsb.as<int32_t>(), wns.as<int32_t>(), newc.as<int32_t>());
+                           n, leaf_max, width, sah ? rounds : 0, wsb.as<int32_t>(), wns.as<int32_t>(),
+                           newc.as<int32_t>());
         FWN_LAUNCH_CHECK("k_collapse_level");
         exclusive_scan_i32(newc.as<int32_t>(), offs.as<int32_t>(), cnt, scan_tmp, s);
         int32_t next_base = gbase + cnt;

@@ -35,6 +35,8 @@ void check_options(const fwn_options& o)
         fail(FWN_INVALID_ARGUMENT, "launch_mode must be FWN_LAUNCH_GRID or FWN_LAUNCH_PERSISTENT");
     if (o.tree_width != 2 && o.tree_width != 4 && o.tree_width != 8)
         fail(FWN_INVALID_ARGUMENT, "tree_width must be 2, 4 or 8");
+    if (o.tree_builder != FWN_TREE_LBVH && o.tree_builder != FWN_TREE_SAH)
+        fail(FWN_INVALID_ARGUMENT, "tree_builder must be FWN_TREE_LBVH or FWN_TREE_SAH");
 }
 
 namespace {
@@ -161,12 +163,13 @@ int fwn_options_default(fwn_options* opt)
     std::memset(opt, 0, sizeof(*opt));
     opt->accuracy_scale = 2.0f;
     opt->near_fp64_rel = 1e-4f;
-    opt->leaf_max = 8;
+    opt->leaf_max = 1;
     opt->fp64_moment_min = 256;
     opt->device = 0;
     opt->waves_per_block = 1;
     opt->tree_width = 4;
     opt->launch_mode = FWN_LAUNCH_GRID;
+    opt->tree_builder = FWN_TREE_SAH;
     return FWN_OK;
 }
 

@@ -113,6 +113,15 @@ void exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, DBuf& tmp, h
 void build_engine(Engine& e, int32_t n_meshes, const float* V_host, const int64_t* v_off, const int32_t* F_host,
                   const int64_t* f_off);
 
+// SAH topology (fwn_sah.hip): top-down binned box-area splits of each mesh's triangles down to
+// single triangles, in the binary tree layout of the LBVH path: internal nodes [0, n-1), leaf j
+// -> n-1+j for the triangle at sorted position j, `items` = sorted position -> triangle,
+// left/right/first/last per internal node, parent per node (the caller fills -1 first),
+// is_root (the caller fills 0) and root_bin per mesh.
+void build_topology_sah(const float* cen, const float* tbox, int64_t nF, const int64_t* f_off, int M,
+                        int32_t* items, int32_t* left, int32_t* right, int32_t* parent, int32_t* first,
+                        int32_t* last, uint8_t* is_root, int32_t* root_bin, hipStream_t s);
+
 // queries (fwn_query.hip)
 void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t nQ, float* omega, uint8_t* inside,
                  int32_t* stats, bool device_ptrs);

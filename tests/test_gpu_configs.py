@@ -5,7 +5,11 @@ size-independent properties.
       restatement of the reference engine (max |dw| <= 4e-3, mean <= 1e-3), a strided sample vs
       the FP64 brute force, and the closed-sphere property (inside for |q| < 0.99, outside > 1.01).
   C3  64 meshes, ~4.2M facets, ~19.2M points in one launch: three meshes bit-identical to their
-      own single-mesh engines, and a sample of every fourth mesh vs the FP64 brute force.
+      own single-mesh engines; on a sample of every fourth mesh, the engine (at its defaults the
+      restated reference's tree) equals the CPU traversal of that tree within 2e-5 in w (99 % of
+      the points; the rest sit on a slot's beta boundary within rounding), and its
+      error against the FP64 brute force is no larger than the restated engine's own plus 1e-4
+      (the reference algorithm reaches ~5e-3 on the coarser tori of this batch).
   C4  200,000 F closed displaced sphere, 4,000,000 points 1e-7..1e-2 x diag off the surface:
       inside flags equal the side of the offset for offsets in [1e-6, 1e-3] x diag (the surface is
       star-shaped, a short normal offset cannot cross another sheet), and a sample vs FP64.
@@ -38,7 +42,7 @@ def test_c1_icosphere_million_points(oracle):
     assert ins[r < 0.99].all() and not ins[r > 1.01].any()
 
 
-def test_c3_batched_full_size():
+def test_c3_batched_full_size(oracle):
     meshes, pts = W.batched_workload()
     mm = MultiFastWindingNumber([(m.vertices, m.facets) for m in meshes])
     om = mm.solid_angle_batch(pts)
@@ -48,11 +52,17 @@ def test_c3_batched_full_size():
         single = FastWindingNumber((meshes[i].vertices, meshes[i].facets))
         np.testing.assert_array_equal(om[off[i]:off[i + 1]], single.solid_angle_batch(pts[i]))
     for i in range(0, 64, 4):
-        single = FastWindingNumber((meshes[i].vertices, meshes[i].facets))
+        V, F = meshes[i].f32()
+        single = FastWindingNumber((V, F))
         sub = pts[i][:: max(1, pts[i].shape[0] // 256)]
         w = single.solid_angle_batch(sub) / FOUR_PI
         wb = single.bruteforce_f64(sub) / FOUR_PI
-        assert np.abs(w - wb).max() <= 4e-3, i
+        R = oracle.Restated(V, F)
+        tau = float(single.export_tree()[3][0])
+        wr = oracle.tree_eval(R.slots, 4, 0, R.root_ns, R.tris, sub, 2.0, tau) / FOUR_PI
+        d = np.abs(w - wr)  # rare points on a slot's beta boundary flip by one far term
+        assert np.mean(d <= 2e-5) >= 0.99 and d.max() <= 1e-3, i
+        assert np.abs(w - wb).max() <= np.abs(wr - wb).max() + 1e-4, i
 
 
 def test_c4_near_surface_full_size():

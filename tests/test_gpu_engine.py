@@ -36,18 +36,19 @@ def slot_view(slots):
 
 def test_sorted_triangle_order_is_morton_order(oracle, ico5):
     V, F = ico5
-    e = FastWindingNumber((V, F))
+    e = FastWindingNumber((V, F), tree_builder=L.FWN_TREE_LBVH)
     _, tris, _, _ = e.export_tree()
     got = tris.reshape(-1, 12)[:, 9].astype(np.int64)
     want, _ = oracle.lbvh_sorted_order([V], [F])
     np.testing.assert_array_equal(got, want)
 
 
+@pytest.mark.parametrize("builder", [L.FWN_TREE_SAH, L.FWN_TREE_LBVH], ids=["sah", "lbvh"])
 @pytest.mark.parametrize("leaf_max,width", [(1, 4), (4, 4), (1, 2), (2, 8)])
-def test_tree_structure_and_moments(oracle, leaf_max, width):
+def test_tree_structure_and_moments(oracle, leaf_max, width, builder):
     m = W.displaced_sphere(12, 5, delete_fraction=0.05)
     V, F = m.f32()
-    e = FastWindingNumber((V, F), leaf_max=leaf_max, tree_width=width)
+    e = FastWindingNumber((V, F), leaf_max=leaf_max, tree_width=width, tree_builder=builder)
     info = e.tree_info()
     assert info["width"] == width
     slots, tris, roots, tau = e.export_tree()
@@ -115,13 +116,14 @@ def _slot_words(rec):
     return f
 
 
-@pytest.mark.parametrize("width,leaf_max", [(4, 1), (2, 1), (8, 1), (4, 4)])
-def test_same_tree_parity_and_counters(oracle, open_mesh, width, leaf_max):
+@pytest.mark.parametrize("builder", [L.FWN_TREE_SAH, L.FWN_TREE_LBVH], ids=["sah", "lbvh"])
+@pytest.mark.parametrize("width,leaf_max", [(4, 1), (2, 1), (8, 1), (4, 4), (4, 8)])
+def test_same_tree_parity_and_counters(oracle, open_mesh, width, leaf_max, builder):
     V, F = open_mesh
     lo, hi = V.min(0), V.max(0)
     q = np.concatenate([W.uniform_queries(lo, hi, 6000, 11, 1.3),
                         W.near_surface_queries(W.Mesh(V.astype(np.float64), F.astype(np.int64)), 2000, 12)])
-    e = FastWindingNumber((V, F), tree_width=width, leaf_max=leaf_max)
+    e = FastWindingNumber((V, F), tree_width=width, leaf_max=leaf_max, tree_builder=builder)
     om, st = e.query_stats(q)
     om2 = e.solid_angle_batch(q)
     np.testing.assert_array_equal(om, om2)  # the counting variant computes the same values
@@ -233,6 +235,8 @@ def test_invalid_inputs_raise():
         FastWindingNumber((V, np.array([[0, 1, 2, 0]], np.int32)))
     with pytest.raises(L.FWNError):
         FastWindingNumber((V, np.array([[0, 1, 2]], np.int32)), leaf_max=9)
+    with pytest.raises(L.FWNError, match="tree_builder"):
+        FastWindingNumber((V, np.array([[0, 1, 2]], np.int32)), tree_builder=2)
     with pytest.raises(LagrangeError):
         FastWindingNumber().solid_angle([0, 0, 0])
 
@@ -246,12 +250,13 @@ def test_multi_mesh_equals_single_engines():
     for i, m in enumerate(meshes):
         lo, hi = (m.bbox() if m.num_facets else (np.full(3, -1.0), np.full(3, 1.0)))
         pts.append(W.uniform_queries(lo, hi, [1000, 77, 64, 5, 2500][i], 50 + i, 1.3))
-    mm = MultiFastWindingNumber([(m.vertices, m.facets) for m in meshes])
-    om = mm.solid_angle_batch(pts)
-    off = np.cumsum([0] + [p.shape[0] for p in pts])
-    for i, m in enumerate(meshes):
-        single = FastWindingNumber((m.vertices, m.facets)).solid_angle_batch(pts[i])
-        np.testing.assert_array_equal(om[off[i]:off[i + 1]], single)
+    for builder in (L.FWN_TREE_SAH, L.FWN_TREE_LBVH):
+        mm = MultiFastWindingNumber([(m.vertices, m.facets) for m in meshes], tree_builder=builder)
+        om = mm.solid_angle_batch(pts)
+        off = np.cumsum([0] + [p.shape[0] for p in pts])
+        for i, m in enumerate(meshes):
+            single = FastWindingNumber((m.vertices, m.facets), tree_builder=builder).solid_angle_batch(pts[i])
+            np.testing.assert_array_equal(om[off[i]:off[i + 1]], single)
 
 
 def test_torch_device_pointers(ico5):

@@ -41,7 +41,9 @@ def test_default_options_match_reference_defaults():
     assert o.accuracy_scale == 2.0  # the reference passes no accuracy argument (default 2)
     assert o.near_fp64_rel == pytest.approx(1e-4)
     assert o.tree_width == 4  # the reference's 4-ary tree
-    assert o.leaf_max == 8 and o.waves_per_block == 1  # measured fastest; same criterion and accuracy
+    assert o.waves_per_block == 1  # measured fastest; same criterion and accuracy
+    # the reference's box-area 4-ary tree layout with single-triangle items
+    assert o.tree_builder == L.FWN_TREE_SAH and o.leaf_max == 1
 
 
 def test_errors_without_device_or_with_bad_arguments():
