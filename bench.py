@@ -500,6 +500,8 @@ def main():
     ap.add_argument("--leaf-max", type=int, default=1,
                     help="leaf bucket size (default 1: the engine's default, the reference's tree layout)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tuned", action="store_true",
+                    help="skip the tuner's engine beside the headline (keeps a profile to the default engine)")
     ap.add_argument("--cpu-stride", type=int, default=1)
     ap.add_argument("--extras", default="batched,near,voxel,examples,closest,io,accuracy,surrogate",
                     help="comma list of extra workloads reported beside the headline: batched, near, voxel, "
@@ -550,17 +552,20 @@ def main():
     value = n * args.steps / dt
 
     # the tuner's engine for the same points (leaf size x launch shape), timed the same way
-    tuned = FastWindingNumber.tuned((V, F), qd, (1, 2, 4, 8))
-    for _ in range(args.warmup):
-        tuned.solid_angle_batch(qd, with_inside=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        om_t, _ = tuned.solid_angle_batch(qd, with_inside=True)
-    torch.cuda.synchronize()
-    dt_t = time.perf_counter() - t0
-    tuned_rec = {"leaf_max": tuned.tuning["leaf_max"], "queries_per_s": n * args.steps / dt_t,
-                 "ms_per_step": dt_t * 1e3 / args.steps, "chosen_by": "fwn_create_tuned over leaf sizes 1, 2, 4, 8"}
+    tuned = om_t = tuned_rec = None
+    if not args.no_tuned:
+        tuned = FastWindingNumber.tuned((V, F), qd, (1, 2, 4, 8))
+        for _ in range(args.warmup):
+            tuned.solid_angle_batch(qd, with_inside=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            om_t, _ = tuned.solid_angle_batch(qd, with_inside=True)
+        torch.cuda.synchronize()
+        dt_t = time.perf_counter() - t0
+        tuned_rec = {"leaf_max": tuned.tuning["leaf_max"], "queries_per_s": n * args.steps / dt_t,
+                     "ms_per_step": dt_t * 1e3 / args.steps,
+                     "chosen_by": "fwn_create_tuned over leaf sizes 1, 2, 4, 8"}
 
     # algorithmic work of the traversal on this tree and criterion (all points, one extra launch)
     _, st = e.query_stats(q)
@@ -578,9 +583,10 @@ def main():
     err = np.abs(w - wb)
     ins_ok = ins.cpu().numpy()[idx] == (wb > 0.5)
     amb = np.abs(wb - 0.5) <= 1e-3
-    tuned_rec["max_abs_dw"] = float(np.abs(om_t.cpu().numpy()[idx].astype(np.float64) / (4 * np.pi) - wb).max())
-    tuned.close()
-    del om_t
+    if tuned is not None:
+        tuned_rec["max_abs_dw"] = float(np.abs(om_t.cpu().numpy()[idx].astype(np.float64) / (4 * np.pi) - wb).max())
+        tuned.close()
+        del om_t
 
     line = {
         "metric": METRIC,
