@@ -150,6 +150,29 @@ __device__ __forceinline__ float far_field_acc(const SlotRegs& r, float dx, floa
     return fmaf(t, inv3, om);
 }
 
+// Deferred FP64 items are written in chunks of kItemChunk slots: a wave claims a chunk with one
+// atomic on the chunk counter and fills it over several leaves, so the shared counter sees one
+// atomic per 64 items instead of one per leaf event.  Unfilled slots of a chunk are written as
+// empty items (count 0), which the FP64 kernels skip.
+constexpr uint32_t kItemChunk = 64;
+struct ItemChunk {
+    uint32_t base = 0u;         // first item slot of the wave's open chunk
+    uint32_t fill = kItemChunk; // slots used in it (kItemChunk: no chunk open)
+};
+
+__device__ __forceinline__ void pad_chunk(const ItemChunk& ch, int lane, uint32_t* __restrict__ items,
+                                          uint32_t item_cap)
+{
+    if ((uint32_t)lane >= ch.fill) {
+        const uint32_t k = ch.base + (uint32_t)lane;
+        if (k < item_cap) {
+            items[3 * (size_t)k] = 0u;
+            items[3 * (size_t)k + 1] = 0u;
+            items[3 * (size_t)k + 2] = 0u;
+        }
+    }
+}
+
 // One batch of up to 64 points (one per lane) of one mesh, walked through that mesh's tree.
 template <bool STATS, int NB>
 __device__ __forceinline__ void traverse_batch(
@@ -158,7 +181,7 @@ __device__ __forceinline__ void traverse_batch(
     const uint32_t* __restrict__ perm, const int64_t* __restrict__ q_off, const int64_t* __restrict__ b_off,
     const int32_t* __restrict__ mesh_order, int M, float beta2, float* __restrict__ omega,
     uint8_t* __restrict__ inside, int32_t* __restrict__ stats, uint32_t* __restrict__ items,
-    uint32_t* __restrict__ n_items, uint32_t item_cap, int32_t* __restrict__ flag)
+    uint32_t* __restrict__ n_chunks, uint32_t item_cap, int32_t* __restrict__ flag, ItemChunk& ch)
 {
     const int lane = threadIdx.x & 63;
     // batches are numbered over the meshes in mesh_order; b_off is the prefix over that order
@@ -235,11 +258,17 @@ __device__ __forceinline__ void traverse_batch(
                         const uint64_t m64 = __ballot(use64);
                         if (m64 != 0ull) {
                             // deferred: (point, leaf) items, evaluated in FP64 by k_near64
-                            uint32_t base = 0u;
-                            if (lane == 0) base = atomicAdd(n_items, (uint32_t)__popcll(m64));
-                            base = __builtin_amdgcn_readfirstlane(base);
+                            const uint32_t c = (uint32_t)__popcll(m64);
+                            if (ch.fill + c > kItemChunk) {
+                                pad_chunk(ch, lane, items, item_cap);
+                                uint32_t nc = 0u;
+                                if (lane == 0) nc = atomicAdd(n_chunks, 1u);
+                                nc = __builtin_amdgcn_readfirstlane(nc);
+                                ch.base = min(nc, 0xFFFFFFFFu / kItemChunk) * kItemChunk;
+                                ch.fill = 0u;
+                            }
                             if (use64) {
-                                const uint32_t k = base + (uint32_t)__popcll(m64 & lt_mask);
+                                const uint32_t k = ch.base + ch.fill + (uint32_t)__popcll(m64 & lt_mask);
                                 if (k < item_cap) {
                                     items[3 * (size_t)k] = jpos;
                                     items[3 * (size_t)k + 1] = (uint32_t)first;
@@ -247,6 +276,7 @@ __device__ __forceinline__ void traverse_batch(
                                 }
                                 if (STATS) n_t64 += tcnt;
                             }
+                            ch.fill += c;
                         }
                     } else {
                         // the host only launches trees whose stack bound 1 + (width - 1) depth fits;
@@ -291,13 +321,14 @@ __global__ __launch_bounds__(WAVES * 64) void k_traverse(
     const uint32_t* __restrict__ perm, const int64_t* __restrict__ q_off, const int64_t* __restrict__ b_off,
     const int32_t* __restrict__ mesh_order, int M, int64_t n_batches, float beta2, float* __restrict__ omega,
     uint8_t* __restrict__ inside, int32_t* __restrict__ stats, uint32_t* __restrict__ items,
-    uint32_t* __restrict__ n_items, uint32_t item_cap, int32_t* __restrict__ flag, uint32_t* __restrict__ work)
+    uint32_t* __restrict__ n_chunks, uint32_t item_cap, int32_t* __restrict__ flag, uint32_t* __restrict__ work)
 {
+    ItemChunk ch;
     if (!PERSIST) {
         const int64_t batch = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * WAVES + (threadIdx.x >> 6);
         if (batch >= n_batches) return; // whole wave
         traverse_batch<STATS, NB>(batch, slots, width, tris, roots, minfo, Q, perm, q_off, b_off, mesh_order, M, beta2,
-                              omega, inside, stats, items, n_items, item_cap, flag);
+                                  omega, inside, stats, items, n_chunks, item_cap, flag, ch);
     } else {
         for (;;) {
             uint32_t b = 0u;
@@ -305,23 +336,25 @@ __global__ __launch_bounds__(WAVES * 64) void k_traverse(
             b = __builtin_amdgcn_readfirstlane(b);
             if ((int64_t)b >= n_batches) break; // every wave leaves once the counter passes the end
             traverse_batch<STATS, NB>((int64_t)b, slots, width, tris, roots, minfo, Q, perm, q_off, b_off, mesh_order, M,
-                                  beta2, omega, inside, stats, items, n_items, item_cap, flag);
+                                      beta2, omega, inside, stats, items, n_chunks, item_cap, flag, ch);
         }
     }
+    pad_chunk(ch, threadIdx.x & 63, items, item_cap); // close the wave's open chunk
 }
 
 // FP64 solid angles of the deferred (point, leaf) items, summed per point in 2^-40 fixed
 // point: integer addition is associative, so the per-point sum does not depend on the order
 // in which items are processed.
 __global__ __launch_bounds__(256) void k_near64(const uint32_t* __restrict__ items,
-                                                const uint32_t* __restrict__ n_items, uint32_t item_cap,
+                                                const uint32_t* __restrict__ n_chunks, uint32_t item_cap,
                                                 const uint32_t* __restrict__ tris, const float* __restrict__ Q,
                                                 const uint32_t* __restrict__ perm,
                                                 unsigned long long* __restrict__ acc)
 {
-    const uint32_t n = min(*n_items, item_cap);
+    const uint32_t n = (uint32_t)min((uint64_t)*n_chunks * kItemChunk, (uint64_t)item_cap);
     for (uint32_t k = blockIdx.x * 256u + threadIdx.x; k < n; k += gridDim.x * 256u) {
         const uint32_t j = items[3 * (size_t)k], first = items[3 * (size_t)k + 1], cnt = items[3 * (size_t)k + 2];
+        if (cnt == 0u) continue; // unfilled slot of a chunk
         const uint32_t orig = perm ? perm[j] : j;
         const float qx = Q[3 * (size_t)orig], qy = Q[3 * (size_t)orig + 1], qz = Q[3 * (size_t)orig + 2];
         double s = 0.0;
@@ -334,13 +367,14 @@ __global__ __launch_bounds__(256) void k_near64(const uint32_t* __restrict__ ite
 // omega = fp32 sum + FP64 near-field sum, once per point: the item that takes the accumulator
 // (atomic exchange with 0) does the update, which also leaves the accumulator zeroed.
 __global__ __launch_bounds__(256) void k_near64_combine(const uint32_t* __restrict__ items,
-                                                        const uint32_t* __restrict__ n_items, uint32_t item_cap,
+                                                        const uint32_t* __restrict__ n_chunks, uint32_t item_cap,
                                                         const uint32_t* __restrict__ perm,
                                                         unsigned long long* __restrict__ acc,
                                                         float* __restrict__ omega, uint8_t* __restrict__ inside)
 {
-    const uint32_t n = min(*n_items, item_cap);
+    const uint32_t n = (uint32_t)min((uint64_t)*n_chunks * kItemChunk, (uint64_t)item_cap);
     for (uint32_t k = blockIdx.x * 256u + threadIdx.x; k < n; k += gridDim.x * 256u) {
+        if (items[3 * (size_t)k + 2] == 0u) continue; // unfilled slot of a chunk
         const uint32_t j = items[3 * (size_t)k];
         const long long v = (long long)atomicExch(&acc[j], 0ull);
         if (v == 0) continue;
@@ -535,7 +569,7 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
     if (nblk > INT32_MAX) fail(FWN_LIMIT_EXCEEDED, "too many point batches");
     const float beta2 = e.opt.accuracy_scale * e.opt.accuracy_scale;
     int32_t* dflag = e.q_flag.as<int32_t>();
-    uint32_t* dnitems = reinterpret_cast<uint32_t*>(dflag + 1);
+    uint32_t* dnitems = reinterpret_cast<uint32_t*>(dflag + 1); // chunks of FP64 items claimed
     uint32_t* dwork = reinterpret_cast<uint32_t*>(dflag + 2);
     for (int attempt = 0;; ++attempt) {
         hipLaunchKernelGGL(k_zero3, dim3(1), dim3(1), 0, s, dflag);
@@ -558,11 +592,11 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
         FWN_HIP(hipMemcpyAsync(h_fl, dflag, sizeof(h_fl), hipMemcpyDeviceToHost, s));
         FWN_HIP(hipStreamSynchronize(s));
         if (h_fl[0]) fail(FWN_LIMIT_EXCEEDED, "traversal stack overflow");
-        const uint32_t need = (uint32_t)h_fl[1];
-        if ((int64_t)need <= e.item_cap) break;
-        if (attempt > 0 || (int64_t)need > (int64_t)UINT32_MAX / 4)
+        const int64_t need = (int64_t)(uint32_t)h_fl[1] * kItemChunk; // claimed chunks x slots
+        if (need <= e.item_cap) break;
+        if (attempt > 0 || need > (int64_t)UINT32_MAX / 4)
             fail(FWN_LIMIT_EXCEEDED, "FP64 near-field work list too large");
-        e.item_cap = (int64_t)need + need / 4;
+        e.item_cap = need + need / 4;
         e.q_items.alloc(sizeof(uint32_t) * 3 * (size_t)e.item_cap);
     }
     if (!dev) {
