@@ -75,8 +75,15 @@ typedef struct fwn_options {
     int32_t tree_width;       /* slots per tree node: 2, 4 or 8 (default 4) */
     int32_t launch_mode;      /* FWN_LAUNCH_GRID (default) or FWN_LAUNCH_PERSISTENT */
     int32_t tree_builder;     /* FWN_TREE_SAH (default) or FWN_TREE_LBVH */
-    int32_t reserved[3];
+    int32_t moment_pass;      /* FWN_MOMENTS_LEVELS (default) or FWN_MOMENTS_CLIMB; both compute the
+                                 same merges in the same order per node (bit-identical trees) */
+    int32_t reserved[2];
 } fwn_options;
+
+/* bottom-up moment pass of the build.  FWN_MOMENTS_LEVELS: one launch per tree depth, deepest
+ * first (trees deeper than 512 binary levels fall back to the climb).  FWN_MOMENTS_CLIMB: one
+ * thread per leaf climbs to the root; the second child to arrive at a node merges it. */
+enum { FWN_MOMENTS_LEVELS = 0, FWN_MOMENTS_CLIMB = 1 };
 
 /* tree builders.  FWN_TREE_LBVH: Morton-ordered binary radix tree (one sort, one pass); a wide
  * node repeatedly opens its largest non-leaf slot until it has tree_width slots.

@@ -26,6 +26,8 @@ FWN_LAUNCH_GRID = 0
 FWN_LAUNCH_PERSISTENT = 1
 FWN_TREE_LBVH = 0
 FWN_TREE_SAH = 1
+FWN_MOMENTS_LEVELS = 0
+FWN_MOMENTS_CLIMB = 1
 
 MAX_WIDTH = 8
 SLOT_WORDS = 32
@@ -46,7 +48,8 @@ class Options(C.Structure):
         ("tree_width", C.c_int32),
         ("launch_mode", C.c_int32),
         ("tree_builder", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("moment_pass", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 

@@ -12,8 +12,12 @@
 //   k_karras        binary radix tree over the sorted keys (Karras 2012), ties broken by index
 //   k_find_roots    the node covering exactly one mesh's triangles is that mesh's root
 //   k_leaf_records  sorted triangle records + FP32 per-triangle moments
-//   k_bottom_up     arrival-counter walk to the roots: merge two children per node, FP32 for
-//                   small nodes and FP64 for nodes with >= fp64_moment_min triangles
+//   k_node_depth, k_depth_bucket, k_merge_level
+//                   moments level by level from the deepest internal nodes up: each node merges
+//                   its two children, FP32 for small nodes and FP64 for nodes with
+//                   >= fp64_moment_min triangles (kernel boundaries order the levels)
+//   k_bottom_up     the same merges by an arrival-counter walk to the roots, for trees deeper
+//                   than kMaxLevels
 //   k_collapse_*    level by level, each wide node takes the binary node's children and
 //                   repeatedly opens the largest non-leaf slot until it has tree_width slots
 //   k_write_slots   FP64 records -> the 32-word FP32 slot format of fwn.h, plus each slot's box
@@ -377,6 +381,83 @@ __global__ __launch_bounds__(BT) void k_bottom_up(const int32_t* __restrict__ pa
     }
 }
 
+// ---- level-synchronous moments ------------------------------------------------------------------
+constexpr int kMaxLevels = 512; // deeper trees (pathological splits) use the arrival-counter walk
+
+// depth of each internal node below its mesh root; -1 for ids outside every mesh tree (unused
+// ids, nodes above the mesh roots) and for nodes deeper than kMaxLevels (then *deep is set).
+// Per-depth counts are gathered in LDS and added to hist once per block and depth.
+__global__ __launch_bounds__(BT) void k_node_depth(const int32_t* __restrict__ parent,
+                                                   const uint8_t* __restrict__ is_root, int64_t n_int,
+                                                   int32_t* __restrict__ depth, int32_t* __restrict__ hist,
+                                                   int32_t* __restrict__ deep)
+{
+    __shared__ int32_t cnt[kMaxLevels];
+    for (int k = threadIdx.x; k < kMaxLevels; k += BT) cnt[k] = 0;
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
+    if (i < n_int) {
+        int d = 0;
+        int32_t u = (int32_t)i;
+        for (;;) { // at most kMaxLevels steps
+            if (is_root[u]) break;
+            const int32_t p = parent[u];
+            if (p < 0) {
+                d = -1;
+                break;
+            }
+            if (++d >= kMaxLevels) {
+                atomicOr(deep, 1);
+                d = -1;
+                break;
+            }
+            u = p;
+        }
+        depth[i] = d;
+        if (d >= 0) atomicAdd(&cnt[d], 1);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kMaxLevels; k += BT)
+        if (cnt[k] > 0) atomicAdd(&hist[k], cnt[k]);
+}
+
+// node ids grouped by depth: level d occupies order[off[d], off[d] + hist[d]); each block
+// reserves its range per depth with one atomic on cursor[d] (order within a level is free)
+__global__ __launch_bounds__(BT) void k_depth_bucket(const int32_t* __restrict__ depth, int64_t n_int,
+                                                     const int32_t* __restrict__ off, int32_t* __restrict__ cursor,
+                                                     int32_t* __restrict__ order)
+{
+    __shared__ int32_t cnt[kMaxLevels];
+    __shared__ int32_t base[kMaxLevels];
+    for (int k = threadIdx.x; k < kMaxLevels; k += BT) cnt[k] = 0;
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
+    const int d = (i < n_int) ? depth[i] : -1;
+    int r = 0;
+    if (d >= 0) r = atomicAdd(&cnt[d], 1);
+    __syncthreads();
+    for (int k = threadIdx.x; k < kMaxLevels; k += BT)
+        if (cnt[k] > 0) base[k] = off[k] + atomicAdd(&cursor[k], cnt[k]);
+    __syncthreads();
+    if (d >= 0) order[base[d] + r] = (int32_t)i;
+}
+
+__global__ __launch_bounds__(BT) void k_merge_level(const int32_t* __restrict__ nodes, int32_t cnt,
+                                                    const int32_t* __restrict__ left,
+                                                    const int32_t* __restrict__ right,
+                                                    const int32_t* __restrict__ first,
+                                                    const int32_t* __restrict__ last, int32_t fp64_min,
+                                                    double* rec)
+{
+    const int i = blockIdx.x * BT + threadIdx.x;
+    if (i >= cnt) return;
+    const int32_t p = nodes[i];
+    const double* a = rec + (int64_t)left[p] * kRecWords;
+    const double* b = rec + (int64_t)right[p] * kRecWords;
+    if (last[p] - first[p] + 1 >= fp64_min) merge2<double>(a, b, rec + (int64_t)p * kRecWords);
+    else merge2<float>(a, b, rec + (int64_t)p * kRecWords);
+}
+
 // ---- collapse into wide nodes -----------------------------------------------------------------
 __device__ __forceinline__ int32_t node_count(int32_t u, int64_t n, const int32_t* first, const int32_t* last)
 {
@@ -693,17 +774,59 @@ void build_engine(Engine& e, int32_t M, const float* V_host, const int64_t* v_of
 
     // ---- moments ----
     rec.alloc(sizeof(double) * kRecWords * nnodes);
-    visits.alloc(sizeof(int32_t) * std::max<int64_t>(n - 1, 1));
-    hipLaunchKernelGGL(k_fill_i32, dim3(grid_blocks(std::max<int64_t>(n - 1, 1), 256)), dim3(256), 0, s,
-                       visits.as<int32_t>(), std::max<int64_t>(n - 1, 1), 0);
     hipLaunchKernelGGL(k_leaf_records, dim3(grid_blocks(n, BT)), dim3(BT), 0, s, vp, n, dV.as<float>(),
                        dF.as<int32_t>(), dfo.as<int64_t>(), dvo.as<int64_t>(), M, e.tris.as<uint32_t>(),
                        rec.as<double>());
     FWN_LAUNCH_CHECK("k_leaf_records");
-    hipLaunchKernelGGL(k_bottom_up, dim3(grid_blocks(n, BT)), dim3(BT), 0, s, parent.as<int32_t>(),
-                       left.as<int32_t>(), right.as<int32_t>(), first.as<int32_t>(), last.as<int32_t>(),
-                       isroot.as<uint8_t>(), n, e.opt.fp64_moment_min, rec.as<double>(), visits.as<int32_t>());
-    FWN_LAUNCH_CHECK("k_bottom_up");
+    if (n > 1) {
+        const int64_t n_int = n - 1;
+        DBuf depth, lvl_hist, lvl_off, order;
+        depth.alloc(sizeof(int32_t) * n_int);
+        lvl_hist.alloc(sizeof(int32_t) * (2 * kMaxLevels + 1)); // hist[kMaxLevels], cursor[kMaxLevels], deep
+        lvl_off.alloc(sizeof(int32_t) * kMaxLevels);
+        order.alloc(sizeof(int32_t) * n_int);
+        int32_t* hist_d = lvl_hist.as<int32_t>();
+        hipLaunchKernelGGL(k_fill_i32, dim3(grid_blocks(2 * kMaxLevels + 1, 256)), dim3(256), 0, s, hist_d,
+                           (int64_t)(2 * kMaxLevels + 1), 0);
+        hipLaunchKernelGGL(k_node_depth, dim3(grid_blocks(n_int, BT)), dim3(BT), 0, s, parent.as<int32_t>(),
+                           isroot.as<uint8_t>(), n_int, depth.as<int32_t>(), hist_d, hist_d + 2 * kMaxLevels);
+        FWN_LAUNCH_CHECK("k_node_depth");
+        std::vector<int32_t> hh(kMaxLevels + 1);
+        FWN_HIP(hipMemcpyAsync(hh.data(), hist_d, sizeof(int32_t) * kMaxLevels, hipMemcpyDeviceToHost, s));
+        FWN_HIP(hipMemcpyAsync(&hh[kMaxLevels], hist_d + 2 * kMaxLevels, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        FWN_HIP(hipStreamSynchronize(s));
+        if (hh[kMaxLevels] == 0 && e.opt.moment_pass == FWN_MOMENTS_LEVELS) {
+            std::vector<int32_t> off(kMaxLevels, 0);
+            int levels = 0;
+            for (int d = 0, acc = 0; d < kMaxLevels; ++d) {
+                off[d] = acc;
+                acc += hh[d];
+                if (hh[d] > 0) levels = d + 1;
+            }
+            FWN_HIP(hipMemcpyAsync(lvl_off.p, off.data(), sizeof(int32_t) * kMaxLevels, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_depth_bucket, dim3(grid_blocks(n_int, BT)), dim3(BT), 0, s, depth.as<int32_t>(),
+                               n_int, lvl_off.as<int32_t>(), hist_d + kMaxLevels, order.as<int32_t>());
+            FWN_LAUNCH_CHECK("k_depth_bucket");
+            for (int d = levels - 1; d >= 0; --d) {
+                if (hh[d] == 0) continue;
+                hipLaunchKernelGGL(k_merge_level, dim3(grid_blocks(hh[d], BT)), dim3(BT), 0, s,
+                                   order.as<int32_t>() + off[d], hh[d], left.as<int32_t>(), right.as<int32_t>(),
+                                   first.as<int32_t>(), last.as<int32_t>(), e.opt.fp64_moment_min, rec.as<double>());
+                FWN_LAUNCH_CHECK("k_merge_level");
+            }
+            // the host vectors above are read by copies already waited for; off by the bucket kernel
+            FWN_HIP(hipStreamSynchronize(s));
+        } else {
+            visits.alloc(sizeof(int32_t) * n_int);
+            hipLaunchKernelGGL(k_fill_i32, dim3(grid_blocks(n_int, 256)), dim3(256), 0, s, visits.as<int32_t>(),
+                               n_int, 0);
+            hipLaunchKernelGGL(k_bottom_up, dim3(grid_blocks(n, BT)), dim3(BT), 0, s, parent.as<int32_t>(),
+                               left.as<int32_t>(), right.as<int32_t>(), first.as<int32_t>(), last.as<int32_t>(),
+                               isroot.as<uint8_t>(), n, e.opt.fp64_moment_min, rec.as<double>(),
+                               visits.as<int32_t>());
+            FWN_LAUNCH_CHECK("k_bottom_up");
+        }
+    }
 
     // ---- collapse ----
     std::vector<int32_t> rb(M);

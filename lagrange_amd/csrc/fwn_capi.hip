@@ -37,6 +37,8 @@ void check_options(const fwn_options& o)
         fail(FWN_INVALID_ARGUMENT, "tree_width must be 2, 4 or 8");
     if (o.tree_builder != FWN_TREE_LBVH && o.tree_builder != FWN_TREE_SAH)
         fail(FWN_INVALID_ARGUMENT, "tree_builder must be FWN_TREE_LBVH or FWN_TREE_SAH");
+    if (o.moment_pass != FWN_MOMENTS_LEVELS && o.moment_pass != FWN_MOMENTS_CLIMB)
+        fail(FWN_INVALID_ARGUMENT, "moment_pass must be FWN_MOMENTS_LEVELS or FWN_MOMENTS_CLIMB");
 }
 
 namespace {
@@ -170,6 +172,7 @@ int fwn_options_default(fwn_options* opt)
     opt->tree_width = 4;
     opt->launch_mode = FWN_LAUNCH_GRID;
     opt->tree_builder = FWN_TREE_SAH;
+    opt->moment_pass = FWN_MOMENTS_LEVELS;
     return FWN_OK;
 }
 

@@ -346,3 +346,20 @@ def test_build_tuner_keeps_the_fastest_leaf_size():
     np.testing.assert_array_equal(single.solid_angle_batch(pts[0]), ref.solid_angle_batch(pts[0]))
     with pytest.raises(L.FWNError):
         FastWindingNumber.tuned((meshes[0].vertices, meshes[0].facets), pts[0], (3, 9))
+
+
+@pytest.mark.parametrize("builder", [L.FWN_TREE_SAH, L.FWN_TREE_LBVH], ids=["sah", "lbvh"])
+def test_moment_passes_give_identical_trees(open_mesh, builder):
+    """The level-synchronous moment pass and the arrival-counter climb merge the same two
+    children per node with the same arithmetic: the exported trees are bit-identical."""
+    V, F = open_mesh
+    for leaf in (1, 8):
+        a = FastWindingNumber((V, F), tree_builder=builder, leaf_max=leaf, moment_pass=L.FWN_MOMENTS_LEVELS)
+        b = FastWindingNumber((V, F), tree_builder=builder, leaf_max=leaf, moment_pass=L.FWN_MOMENTS_CLIMB)
+        for x, y in zip(a.export_tree(), b.export_tree()):
+            np.testing.assert_array_equal(x, y)
+    meshes = [W.icosphere(6).f32(), W.torus(40, 12, seed=3, amplitude=0.2).f32(), (V, F)]
+    ma = MultiFastWindingNumber(meshes, tree_builder=builder, moment_pass=L.FWN_MOMENTS_LEVELS)
+    mb = MultiFastWindingNumber(meshes, tree_builder=builder, moment_pass=L.FWN_MOMENTS_CLIMB)
+    for x, y in zip(ma.export_tree(), mb.export_tree()):
+        np.testing.assert_array_equal(x, y)
