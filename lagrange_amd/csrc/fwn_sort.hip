@@ -8,7 +8,8 @@
 //   digits:  one block scans the 256 digit totals
 //   scatter: each block re-reads its tile in 16 rounds of 256 keys; inside a wave the rank of a
 //            key among equal digits comes from 8 ballots, across the block's 4 waves from LDS
-//            counts, and the running per-digit offset keeps the order stable.
+//            counts, and the running per-digit offset keeps the order stable; the tile is
+//            placed in LDS in digit order first, then written out in coalesced runs.
 #include "fwn_internal.h"
 
 namespace fwn {
@@ -76,6 +77,9 @@ __global__ __launch_bounds__(256) void k_rs_digits(int32_t* __restrict__ rowsum)
     rowsum[threadIdx.x] = part[threadIdx.x] - v;
 }
 
+// The tile is first ordered by digit in LDS (stable), then written out: consecutive threads
+// write consecutive positions of a digit's run, so the global stores coalesce.  The output is
+// the same as writing each key straight to its position.
 template <typename KeyT>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const KeyT* __restrict__ kin,
                                                            const uint32_t* __restrict__ vin,
@@ -83,43 +87,72 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const KeyT* __restric
                                                            int64_t n, int shift, const int32_t* __restrict__ hist,
                                                            const int32_t* __restrict__ digit_base, int nblocks)
 {
-    __shared__ int32_t running[256];
+    __shared__ int32_t gbase[256];   // global position of this tile's first key of each digit
+    __shared__ int32_t toff[256];    // tile-local start of each digit's run
+    __shared__ int32_t lrun[256];    // keys of each digit already placed in earlier rounds
     __shared__ int32_t wc[RS_WAVES][256];
+    __shared__ KeyT lk[RS_TILE];
+    __shared__ uint32_t lv[RS_TILE];
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wave = tid / kWave;
-    running[tid] = hist[(int64_t)tid * nblocks + blockIdx.x] + digit_base[tid];
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    const int tile_n = (int)((n - base) < RS_TILE ? (n - base) : RS_TILE);
+    // this tile's count of digit tid: the next block's exclusive offset (or the digit total) minus ours
+    const int32_t mine = hist[(int64_t)tid * nblocks + blockIdx.x];
+    int32_t next;
+    if (blockIdx.x + 1 < nblocks) next = hist[(int64_t)tid * nblocks + blockIdx.x + 1];
+    else next = ((tid < 255) ? digit_base[tid + 1] : (int32_t)n) - digit_base[tid];
+    const int32_t cnt = next - mine;
+    gbase[tid] = mine + digit_base[tid];
+    // exclusive scan of the tile's digit counts (Hillis-Steele in LDS)
+    toff[tid] = cnt;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int32_t t = (tid >= off) ? toff[tid - off] : 0;
+        __syncthreads();
+        toff[tid] += t;
+        __syncthreads();
+    }
+    toff[tid] -= cnt;
+    lrun[tid] = 0;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    int64_t base = (int64_t)blockIdx.x * RS_TILE;
     for (int r = 0; r < RS_ROUNDS; ++r) {
-        if (base + r * RS_THREADS >= n) break; // uniform across the block
+        if (r * RS_THREADS >= tile_n) break; // uniform across the block
         for (int k = tid; k < RS_WAVES * 256; k += RS_THREADS) (&wc[0][0])[k] = 0;
-        int64_t i = base + r * RS_THREADS + tid;
-        bool valid = i < n;
-        KeyT key = valid ? kin[i] : (KeyT)0;
-        uint32_t val = valid ? vin[i] : 0u;
-        uint32_t d = (uint32_t)(key >> shift) & 255u;
+        const int li = r * RS_THREADS + tid;
+        const bool valid = li < tile_n;
+        const KeyT key = valid ? kin[base + li] : (KeyT)0;
+        const uint32_t val = valid ? vin[base + li] : 0u;
+        const uint32_t d = (uint32_t)(key >> shift) & 255u;
         uint64_t peers = __ballot(valid);
         for (int b = 0; b < 8; ++b) {
-            bool bit = (d >> b) & 1u;
-            uint64_t m = __ballot(bit);
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = __ballot(bit);
             peers &= bit ? m : ~m;
         }
-        int rank = __popcll(peers & lt);
+        const int rank = __popcll(peers & lt);
         __syncthreads();
         if (valid && rank == 0) wc[wave][d] = __popcll(peers);
         __syncthreads();
         if (valid) {
-            int32_t pos = running[d] + rank;
+            int32_t pos = toff[d] + lrun[d] + rank;
             for (int w = 0; w < wave; ++w) pos += wc[w][d];
-            kout[pos] = key;
-            vout[pos] = val;
+            lk[pos] = key;
+            lv[pos] = val;
         }
         __syncthreads();
         int32_t add = 0;
         for (int w = 0; w < RS_WAVES; ++w) add += wc[w][tid];
-        running[tid] += add;
+        lrun[tid] += add;
         __syncthreads();
+    }
+    for (int t = tid; t < tile_n; t += RS_THREADS) {
+        const KeyT key = lk[t];
+        const uint32_t d = (uint32_t)(key >> shift) & 255u;
+        const int32_t pos = gbase[d] + (t - toff[d]);
+        kout[pos] = key;
+        vout[pos] = lv[t];
     }
 }
 
