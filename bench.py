@@ -551,6 +551,17 @@ def main():
     ms_per_step = dt * 1e3 / args.steps
     value = n * args.steps / dt
 
+    # PCIe-inclusive rate of the same call through host pointers (points uploaded, results copied
+    # back to pageable host arrays): what a caller holding host buffers sees; never `value`
+    e.solid_angle_batch(q, with_inside=True)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        e.solid_angle_batch(q, with_inside=True)
+    dt_h = (time.perf_counter() - t0) / 3
+    host_rec = {"queries_per_s": n / dt_h, "ms_per_call": dt_h * 1e3,
+                "bytes_moved": int(n * (12 + 4 + 1)),
+                "note": "host (pageable) arrays in and out through the C ABI, copies included"}
+
     # the tuner's engine for the same points (leaf size x launch shape), timed the same way
     tuned = om_t = tuned_rec = None
     if not args.no_tuned:
@@ -639,6 +650,7 @@ def main():
             "inside_mismatch_excl_band": int((~ins_ok & ~amb).sum()),
         },
         "tuned": tuned_rec,
+        "host_arrays": host_rec,
     }
     extras = [x for x in args.extras.split(",") if x and x != "none"]
     if "batched" in extras:
