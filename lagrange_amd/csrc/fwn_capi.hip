@@ -65,6 +65,7 @@ void destroy_engine(Engine* e)
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     for (auto& ev : e->ev)
         if (ev) (void)hipEventDestroy(ev);
+    if (e->h_stage) (void)hipHostFree(e->h_stage);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }

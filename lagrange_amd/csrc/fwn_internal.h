@@ -81,8 +81,8 @@ struct Engine {
     std::vector<int64_t> h_f_off;
 
     // query scratch (grown on demand)
-    DBuf q_keys[2], q_vals[2], q_hist, q_rowsum, q_in, q_om, q_in8, q_off, q_boff, q_stats, q_flag;
-    DBuf q_bf, q_order;
+    DBuf q_keys[2], q_vals[2], q_hist, q_rowsum, q_in, q_om, q_in8, q_stats, q_flag;
+    DBuf q_bf;
     DBuf v_pts, v_om, v_in, v_ijk; // query points generated on the device and their outputs
     DBuf c_V, c_F, c_err, c_out, c_pos, c_scan, c_outp; // orientation / sampling scratch
     DBuf d_d2, d_tri, d_cp;                               // closest-point outputs for host callers
@@ -92,6 +92,12 @@ struct Engine {
     DBuf q_items, q_acc;
     int64_t item_cap = 0;
     int64_t acc_cap = 0;
+
+    // pinned host staging for small calls: metadata, points and results in one copy each way
+    // (reused only after the call that filled it has synchronised its stream)
+    void* h_stage = nullptr;
+    size_t h_stage_bytes = 0;
+    DBuf q_meta; // device copy of the staged metadata (and small host batches' points)
 
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double last_traverse_ms = 0.0;

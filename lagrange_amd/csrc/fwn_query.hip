@@ -19,6 +19,8 @@
 // use the FP64 van Oosterom-Strackee formula.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include <algorithm>
 #include <cmath>
 
@@ -173,13 +175,14 @@ __device__ __forceinline__ void pad_chunk(const ItemChunk& ch, int lane, uint32_
     }
 }
 
-// One batch of up to 64 points (one per lane) of one mesh, walked through that mesh's tree.
+// One batch of up to bpts <= 64 points (one per lane) of one mesh, walked through that mesh's
+// tree (bpts is 64 except in small calls, which spread their points over more waves).
 template <bool STATS, int NB>
 __device__ __forceinline__ void traverse_batch(
     int64_t batch, const uint32_t* __restrict__ slots, int width, const uint32_t* __restrict__ tris,
     const int32_t* __restrict__ roots, const float* __restrict__ minfo, const float* __restrict__ Q,
     const uint32_t* __restrict__ perm, const int64_t* __restrict__ q_off, const int64_t* __restrict__ b_off,
-    const int32_t* __restrict__ mesh_order, int M, float beta2, float* __restrict__ omega,
+    const int32_t* __restrict__ mesh_order, int M, int bpts, float beta2, float* __restrict__ omega,
     uint8_t* __restrict__ inside, int32_t* __restrict__ stats, uint32_t* __restrict__ items,
     uint32_t* __restrict__ n_chunks, uint32_t item_cap, int32_t* __restrict__ flag, ItemChunk& ch)
 {
@@ -187,9 +190,9 @@ __device__ __forceinline__ void traverse_batch(
     // batches are numbered over the meshes in mesh_order; b_off is the prefix over that order
     const int pos = mesh_of64(b_off, M, batch);
     const int m = mesh_order[pos];
-    const int64_t start = q_off[m] + (batch - b_off[pos]) * 64;
+    const int64_t start = q_off[m] + (batch - b_off[pos]) * bpts;
     const int64_t cnt64 = q_off[m + 1] - start;
-    const int cnt = cnt64 < 64 ? (int)cnt64 : 64;
+    const int cnt = cnt64 < bpts ? (int)cnt64 : bpts;
     const bool valid = lane < cnt;
     const uint32_t jpos = (uint32_t)(start + lane); // sorted position of this lane's point
     const uint32_t orig = valid ? (perm ? perm[jpos] : jpos) : 0u;
@@ -319,7 +322,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_traverse(
     const uint32_t* __restrict__ slots, int width, const uint32_t* __restrict__ tris,
     const int32_t* __restrict__ roots, const float* __restrict__ minfo, const float* __restrict__ Q,
     const uint32_t* __restrict__ perm, const int64_t* __restrict__ q_off, const int64_t* __restrict__ b_off,
-    const int32_t* __restrict__ mesh_order, int M, int64_t n_batches, float beta2, float* __restrict__ omega,
+    const int32_t* __restrict__ mesh_order, int M, int64_t n_batches, int bpts, float beta2, float* __restrict__ omega,
     uint8_t* __restrict__ inside, int32_t* __restrict__ stats, uint32_t* __restrict__ items,
     uint32_t* __restrict__ n_chunks, uint32_t item_cap, int32_t* __restrict__ flag, uint32_t* __restrict__ work)
 {
@@ -327,8 +330,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_traverse(
     if (!PERSIST) {
         const int64_t batch = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * WAVES + (threadIdx.x >> 6);
         if (batch >= n_batches) return; // whole wave
-        traverse_batch<STATS, NB>(batch, slots, width, tris, roots, minfo, Q, perm, q_off, b_off, mesh_order, M, beta2,
-                                  omega, inside, stats, items, n_chunks, item_cap, flag, ch);
+        traverse_batch<STATS, NB>(batch, slots, width, tris, roots, minfo, Q, perm, q_off, b_off, mesh_order, M, bpts,
+                                  beta2, omega, inside, stats, items, n_chunks, item_cap, flag, ch);
     } else {
         for (;;) {
             uint32_t b = 0u;
@@ -336,7 +339,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_traverse(
             b = __builtin_amdgcn_readfirstlane(b);
             if ((int64_t)b >= n_batches) break; // every wave leaves once the counter passes the end
             traverse_batch<STATS, NB>((int64_t)b, slots, width, tris, roots, minfo, Q, perm, q_off, b_off, mesh_order, M,
-                                      beta2, omega, inside, stats, items, n_chunks, item_cap, flag, ch);
+                                      bpts, beta2, omega, inside, stats, items, n_chunks, item_cap, flag, ch);
         }
     }
     pad_chunk(ch, threadIdx.x & 63, items, item_cap); // close the wave's open chunk
@@ -460,6 +463,23 @@ int resident_blocks(bool persist, int waves, int banks)
     return nb > 0 ? nb : 1;
 }
 
+// Calls with at most this many points skip the Morton sort (12 small launches that would cost
+// more than the coherence they buy) and move host data through the pinned staging buffer.
+constexpr int64_t kSmallBatch = 2048;
+
+char* host_stage(Engine& e, size_t bytes)
+{
+    if (e.h_stage_bytes < bytes) {
+        if (e.h_stage) FWN_HIP(hipHostFree(e.h_stage));
+        e.h_stage = nullptr;
+        e.h_stage_bytes = 0;
+        const size_t nb = std::max<size_t>(bytes, 1 << 16);
+        FWN_HIP(hipHostMalloc(&e.h_stage, nb, hipHostMallocDefault));
+        e.h_stage_bytes = nb;
+    }
+    return static_cast<char*>(e.h_stage);
+}
+
 } // namespace
 
 void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t nQ, float* omega, uint8_t* inside,
@@ -492,19 +512,44 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
     std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
         return e.h_f_off[a + 1] - e.h_f_off[a] > e.h_f_off[b + 1] - e.h_f_off[b];
     });
+    // points per wave: 64, or fewer for small calls so that their waves spread over the device
+    // (a point's result does not depend on which points share its wave)
+    const bool small = nQ <= kSmallBatch && !stats;
+    const int bpts = small ? (int)std::min<int64_t>(64, std::max<int64_t>(1, (nQ + 1023) / 1024)) : 64;
     std::vector<int64_t> boff(M + 1, 0);
-    for (int k = 0; k < M; ++k) boff[k + 1] = boff[k] + (qoff[order[k] + 1] - qoff[order[k]] + 63) / 64;
+    for (int k = 0; k < M; ++k) boff[k + 1] = boff[k] + (qoff[order[k] + 1] - qoff[order[k]] + bpts - 1) / bpts;
     const int64_t nb = boff[M];
+
+    const bool stage_io = small && !dev; // points in and results out through the pinned stage
+    // metadata (and the points of a small host batch) in one staged copy:
+    // q_off[M+1] (int64), b_off[M+1] (int64), mesh order[M] (int32), then the points at 16 B
+    const size_t meta_b = (sizeof(int64_t) * 2 * (M + 1) + sizeof(int32_t) * M + 15) & ~(size_t)15;
+    const size_t in_b = meta_b + (stage_io ? sizeof(float) * 3 * (size_t)nQ : 0);
+    const size_t out_b = stage_io ? 16 + (sizeof(float) + 1) * (size_t)nQ : 0; // flags, omega, inside
+    char* hs = host_stage(e, std::max(in_b, out_b));
+    std::memcpy(hs, qoff.data(), sizeof(int64_t) * (M + 1));
+    std::memcpy(hs + sizeof(int64_t) * (M + 1), boff.data(), sizeof(int64_t) * (M + 1));
+    std::memcpy(hs + sizeof(int64_t) * 2 * (M + 1), order.data(), sizeof(int32_t) * M);
+    if (stage_io) std::memcpy(hs + meta_b, Q, sizeof(float) * 3 * (size_t)nQ);
+    e.q_meta.reserve(in_b);
+    FWN_HIP(hipMemcpyAsync(e.q_meta.p, hs, in_b, hipMemcpyHostToDevice, s));
+    const int64_t* d_qoff = e.q_meta.as<int64_t>();
+    const int64_t* d_boff = d_qoff + (M + 1);
+    const int32_t* d_order = reinterpret_cast<const int32_t*>(d_qoff + 2 * (M + 1));
 
     const float* dQ = Q;
     float* dOm = omega;
     uint8_t* dIn = inside;
     int32_t* dSt = stats;
     if (!dev) {
-        e.q_in.reserve(sizeof(float) * 3 * nQ);
         e.q_om.reserve(sizeof(float) * nQ);
-        FWN_HIP(hipMemcpyAsync(e.q_in.p, Q, sizeof(float) * 3 * nQ, hipMemcpyHostToDevice, s));
-        dQ = e.q_in.as<float>();
+        if (stage_io) {
+            dQ = reinterpret_cast<const float*>(e.q_meta.as<char>() + meta_b);
+        } else {
+            e.q_in.reserve(sizeof(float) * 3 * nQ);
+            FWN_HIP(hipMemcpyAsync(e.q_in.p, Q, sizeof(float) * 3 * nQ, hipMemcpyHostToDevice, s));
+            dQ = e.q_in.as<float>();
+        }
         dOm = e.q_om.as<float>();
         if (inside) {
             e.q_in8.reserve(nQ);
@@ -515,13 +560,7 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
             dSt = e.q_stats.as<int32_t>();
         }
     }
-    e.q_off.reserve(sizeof(int64_t) * (M + 1));
-    e.q_boff.reserve(sizeof(int64_t) * (M + 1));
-    e.q_order.reserve(sizeof(int32_t) * M);
     e.q_flag.reserve(3 * sizeof(int32_t));
-    FWN_HIP(hipMemcpyAsync(e.q_off.p, qoff.data(), sizeof(int64_t) * (M + 1), hipMemcpyHostToDevice, s));
-    FWN_HIP(hipMemcpyAsync(e.q_boff.p, boff.data(), sizeof(int64_t) * (M + 1), hipMemcpyHostToDevice, s));
-    FWN_HIP(hipMemcpyAsync(e.q_order.p, order.data(), sizeof(int32_t) * M, hipMemcpyHostToDevice, s));
 
     for (int k = 0; k < 2; ++k) {
         e.q_keys[k].reserve(sizeof(uint32_t) * nQ);
@@ -543,17 +582,21 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
     }
 
     FWN_HIP(hipEventRecord(e.ev[0], s));
-    uint32_t* kp = e.q_keys[0].as<uint32_t>();
-    uint32_t* vp = e.q_vals[0].as<uint32_t>();
-    uint32_t* kp2 = e.q_keys[1].as<uint32_t>();
-    uint32_t* vp2 = e.q_vals[1].as<uint32_t>();
-    int mesh_bits = 0;
-    while ((1 << mesh_bits) < M) ++mesh_bits;
-    if (mesh_bits > 24) fail(FWN_LIMIT_EXCEEDED, "more than 2^24 meshes in one engine");
-    hipLaunchKernelGGL(k_query_keys, dim3(grid_blocks(nQ, QT)), dim3(QT), 0, s, dQ, nQ, e.q_off.as<int64_t>(), M,
-                       mesh_bits, e.mesh_info.as<float>(), kp, vp);
-    FWN_LAUNCH_CHECK("k_query_keys");
-    radix_sort_pairs32(kp, vp, kp2, vp2, nQ, 8, 32, e.q_hist, e.q_rowsum, s);
+    // sorted position -> input index (nullptr: small calls keep the input order)
+    uint32_t* vp = nullptr;
+    if (!small) {
+        uint32_t* kp = e.q_keys[0].as<uint32_t>();
+        uint32_t* kp2 = e.q_keys[1].as<uint32_t>();
+        uint32_t* vp2 = e.q_vals[1].as<uint32_t>();
+        vp = e.q_vals[0].as<uint32_t>();
+        int mesh_bits = 0;
+        while ((1 << mesh_bits) < M) ++mesh_bits;
+        if (mesh_bits > 24) fail(FWN_LIMIT_EXCEEDED, "more than 2^24 meshes in one engine");
+        hipLaunchKernelGGL(k_query_keys, dim3(grid_blocks(nQ, QT)), dim3(QT), 0, s, dQ, nQ, d_qoff, M, mesh_bits,
+                           e.mesh_info.as<float>(), kp, vp);
+        FWN_LAUNCH_CHECK("k_query_keys");
+        radix_sort_pairs32(kp, vp, kp2, vp2, nQ, 8, 32, e.q_hist, e.q_rowsum, s);
+    }
 
     const int waves = e.opt.waves_per_block > 0 ? e.opt.waves_per_block : 1;
     const bool persist = (e.opt.launch_mode == FWN_LAUNCH_PERSISTENT) && !stats;
@@ -576,21 +619,30 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
         FWN_HIP(hipEventRecord(e.ev[1], s));
         const uint32_t cap = (uint32_t)e.item_cap;
         launch_traverse(stats != nullptr, waves, persist, banks, (unsigned)nblk, s, e.slots.as<uint32_t>(), e.width,
-                        e.tris.as<uint32_t>(), e.roots.as<int32_t>(), e.mesh_info.as<float>(), dQ, vp,
-                        e.q_off.as<int64_t>(), e.q_boff.as<int64_t>(), e.q_order.as<int32_t>(), M, nb, beta2, dOm,
-                        dIn, dSt, e.q_items.as<uint32_t>(), dnitems, cap, dflag, dwork);
+                        e.tris.as<uint32_t>(), e.roots.as<int32_t>(), e.mesh_info.as<float>(), dQ, vp, d_qoff,
+                        d_boff, d_order, M, nb, bpts, beta2, dOm, dIn, dSt, e.q_items.as<uint32_t>(), dnitems, cap,
+                        dflag, dwork);
         FWN_LAUNCH_CHECK("k_traverse");
         FWN_HIP(hipEventRecord(e.ev[2], s));
-        hipLaunchKernelGGL(k_near64, dim3(2048), dim3(256), 0, s, e.q_items.as<uint32_t>(), dnitems, cap,
+        const unsigned g64 = small ? 16u : 2048u; // grid-stride loops: any grid size covers every item
+        hipLaunchKernelGGL(k_near64, dim3(g64), dim3(256), 0, s, e.q_items.as<uint32_t>(), dnitems, cap,
                            e.tris.as<uint32_t>(), dQ, vp, e.q_acc.as<unsigned long long>());
         FWN_LAUNCH_CHECK("k_near64");
-        hipLaunchKernelGGL(k_near64_combine, dim3(2048), dim3(256), 0, s, e.q_items.as<uint32_t>(), dnitems, cap, vp,
+        hipLaunchKernelGGL(k_near64_combine, dim3(g64), dim3(256), 0, s, e.q_items.as<uint32_t>(), dnitems, cap, vp,
                            e.q_acc.as<unsigned long long>(), dOm, dIn);
         FWN_LAUNCH_CHECK("k_near64_combine");
         FWN_HIP(hipEventRecord(e.ev[3], s));
         int32_t h_fl[2] = {0, 0};
-        FWN_HIP(hipMemcpyAsync(h_fl, dflag, sizeof(h_fl), hipMemcpyDeviceToHost, s));
-        FWN_HIP(hipStreamSynchronize(s));
+        if (stage_io) { // flags and results in one wait
+            FWN_HIP(hipMemcpyAsync(hs, dflag, sizeof(h_fl), hipMemcpyDeviceToHost, s));
+            FWN_HIP(hipMemcpyAsync(hs + 16, dOm, sizeof(float) * nQ, hipMemcpyDeviceToHost, s));
+            if (inside) FWN_HIP(hipMemcpyAsync(hs + 16 + sizeof(float) * nQ, dIn, nQ, hipMemcpyDeviceToHost, s));
+            FWN_HIP(hipStreamSynchronize(s));
+            std::memcpy(h_fl, hs, sizeof(h_fl));
+        } else {
+            FWN_HIP(hipMemcpyAsync(h_fl, dflag, sizeof(h_fl), hipMemcpyDeviceToHost, s));
+            FWN_HIP(hipStreamSynchronize(s));
+        }
         if (h_fl[0]) fail(FWN_LIMIT_EXCEEDED, "traversal stack overflow");
         const int64_t need = (int64_t)(uint32_t)h_fl[1] * kItemChunk; // claimed chunks x slots
         if (need <= e.item_cap) break;
@@ -599,7 +651,10 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
         e.item_cap = need + need / 4;
         e.q_items.alloc(sizeof(uint32_t) * 3 * (size_t)e.item_cap);
     }
-    if (!dev) {
+    if (stage_io) {
+        std::memcpy(omega, hs + 16, sizeof(float) * nQ);
+        if (inside) std::memcpy(inside, hs + 16 + sizeof(float) * nQ, nQ);
+    } else if (!dev) {
         FWN_HIP(hipMemcpyAsync(omega, dOm, sizeof(float) * nQ, hipMemcpyDeviceToHost, s));
         if (inside) FWN_HIP(hipMemcpyAsync(inside, dIn, nQ, hipMemcpyDeviceToHost, s));
         if (stats) FWN_HIP(hipMemcpyAsync(stats, dSt, sizeof(int32_t) * 4 * nQ, hipMemcpyDeviceToHost, s));

@@ -363,3 +363,31 @@ def test_moment_passes_give_identical_trees(open_mesh, builder):
     mb = MultiFastWindingNumber(meshes, tree_builder=builder, moment_pass=L.FWN_MOMENTS_CLIMB)
     for x, y in zip(ma.export_tree(), mb.export_tree()):
         np.testing.assert_array_equal(x, y)
+
+
+def test_small_calls_equal_sorted_calls(ico5):
+    """Calls of at most 2,048 points skip the Morton sort and stage host data through pinned
+    memory; a point's result does not depend on which points share its wave, so they equal the
+    same points answered inside a large (sorted) call bit for bit, for host and device arrays."""
+    import torch
+    V, F = ico5
+    e = FastWindingNumber((V, F))
+    rng = np.random.default_rng(11)
+    big = rng.uniform(-1.4, 1.4, (20000, 3)).astype(np.float32)
+    om_big, in_big = e.solid_angle_batch(big, with_inside=True)
+    for lo, hi in ((0, 1), (5, 69), (100, 2148), (3000, 3001)):
+        om, ins = e.solid_angle_batch(big[lo:hi], with_inside=True)
+        np.testing.assert_array_equal(om, om_big[lo:hi])
+        np.testing.assert_array_equal(ins, in_big[lo:hi])
+        omd = e.solid_angle_batch(torch.from_numpy(big[lo:hi]).cuda())
+        np.testing.assert_array_equal(omd.cpu().numpy(), om_big[lo:hi])
+    for k in (0, 7, 1999):
+        assert e.solid_angle(big[k]) == om_big[k]
+        assert e.is_inside(big[k]) == bool(in_big[k])
+    meshes = [W.icosphere(6).f32(), (V, F)]
+    pts = [rng.uniform(-1.3, 1.3, (300, 3)).astype(np.float32), big[:500]]
+    mm = MultiFastWindingNumber(meshes)
+    om_m = mm.solid_angle_batch(pts)
+    np.testing.assert_array_equal(om_m[300:], om_big[:500])
+    np.testing.assert_array_equal(om_m[:300], FastWindingNumber(meshes[0]).solid_angle_batch(
+        np.concatenate([pts[0], rng.uniform(-1.3, 1.3, (5000, 3)).astype(np.float32)]))[:300])
