@@ -91,6 +91,12 @@ void create_common(fwn_engine** out, int32_t M, const float* V, const int64_t* v
         DeviceGuard g(e->device);
         FWN_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
         build_engine(*e, M, V, v_off, F, f_off);
+        if (e->opt.tree_builder == FWN_TREE_SAH && stack_banks(e->width, e->max_depth) == 0) {
+            // an SAH tree too deep for the traversal stack (extreme splits on extreme inputs):
+            // the Morton tree's depth is bounded by its key bits
+            e->opt.tree_builder = FWN_TREE_LBVH;
+            build_engine(*e, M, V, v_off, F, f_off);
+        }
     } catch (...) {
         destroy_engine(e);
         throw;
