@@ -84,3 +84,27 @@ def test_c4_near_surface_full_size():
     wb = e.bruteforce_f64(q[idx]) / FOUR_PI
     err = np.abs(om[idx] / FOUR_PI - wb)
     assert err.max() <= 4e-3 and err.mean() <= 1e-3
+
+
+def test_large_random_soup_both_builders():
+    """A 4,000,000-triangle random soup (overlapping, intersecting, arbitrary orientation) through
+    both tree builders: the build succeeds, both agree with the FP64 brute force on a sample
+    within the order-2 bound, and their is_inside flags agree away from w = 1/2."""
+    rng = np.random.default_rng(42)
+    n = 4_000_000
+    c = rng.uniform(-1.0, 1.0, (n, 3)).astype(np.float32)
+    V = (c[:, None, :] + rng.normal(0.0, 0.01, (n, 3, 3)).astype(np.float32)).reshape(-1, 3)
+    F = np.arange(3 * n, dtype=np.int32).reshape(n, 3)
+    q = rng.uniform(-1.2, 1.2, (200_000, 3)).astype(np.float32)
+    outs = []
+    for builder in (1, 0):
+        e = FastWindingNumber((V, F), tree_builder=builder)
+        om, ins = e.solid_angle_batch(q, with_inside=True)
+        wb = e.bruteforce_f64(q[::4000]) / FOUR_PI
+        err = np.abs(om[::4000] / FOUR_PI - wb)
+        print(f"builder {builder}: build {e.tree_info()['build_ms']:.1f} ms, max |dw| {err.max():.2e}", flush=True)
+        assert err.max() <= 4e-3
+        outs.append((om / FOUR_PI, ins))
+        e.close()
+    far = np.abs(outs[0][0] - 0.5) > 1e-2
+    assert np.array_equal(outs[0][1][far], outs[1][1][far])
