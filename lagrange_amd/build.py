@@ -23,6 +23,13 @@ HIP_FLAGS = [
 ]
 
 
+# per-source additions: the surrogate's MFMA kernels keep their accumulators in VGPRs (the
+# fused MLP's bias/ReLU/BF16 epilogue then reads them without 512 AGPR-to-VGPR moves per tile)
+EXTRA_FLAGS = {
+    "fwn_surrogate.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
+}
+
+
 def sources():
     return sorted(glob.glob(os.path.join(HERE, "csrc", "*.hip")))
 
@@ -43,7 +50,7 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     for src in sources():
         obj = os.path.join(objdir, os.path.basename(src) + ".o")
         objs.append(obj)
-        cmd = [hipcc()] + HIP_FLAGS + inc + ["-c", src, "-o", obj]
+        cmd = [hipcc()] + HIP_FLAGS + EXTRA_FLAGS.get(os.path.basename(src), []) + inc + ["-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))

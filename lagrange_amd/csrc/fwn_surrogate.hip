@@ -440,7 +440,25 @@ constexpr int IW = 4;                 // waves per inference workgroup (one per 
 constexpr int64_t kSortMin = 1 << 16; // batches at least this large are encoded in Morton order
 constexpr int kFrags = 8 + 3 * 32;    // A fragments of the four hidden layers
 
-__device__ __forceinline__ uint32_t pack2(float a, float b) { return (uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16); }
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+// two floats -> packed BF16 (a in the low half), round to nearest even: one v_cvt_pk_bf16_f32 on
+// gfx950, the same values as f2bf for finite inputs
+__device__ __forceinline__ uint32_t pack2(float a, float b)
+{
+    const f32x2_t x = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(x, bf16x2_t));
+}
+typedef short s16x2_t __attribute__((ext_vector_type(2)));
+// ReLU of two packed BF16 values as one packed 16-bit integer max with 0: BF16 bit patterns of
+// negative values (and -0) are negative as int16, so max(pattern, 0) is +0 for them and leaves
+// the rest unchanged; round-to-nearest-even keeps the sign, so relu_pk(pack2(a, b)) equals
+// pack2(max(a, 0), max(b, 0)) for finite a, b
+__device__ __forceinline__ uint32_t relu_pk(uint32_t w)
+{
+    const s16x2_t z = {0, 0};
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2_t, w), z));
+}
 
 // One hidden layer: for each 32-row output tile t, acc = sum_s A(layer, t, s) B[p][s], then bias +
 // ReLU + BF16; registers 8q..8q+7 of tile t become fragment 2t+q of the next layer's input (Bn),
@@ -463,21 +481,27 @@ __device__ __forceinline__ void mlp_layer(const u16x8* __restrict__ sA, int lane
         }
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
-            float v[16];
+            // bias + ReLU, rounded to BF16 in pairs (the next layer's input, or the output layer's)
+            uint32_t w[8];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
+            for (int i = 0; i < 8; ++i) {
+                const int r = 2 * i;
                 const int m = 32 * t + 8 * (r >> 2) + 4 * h + (r & 3);
-                v[r] = bf2f(f2bf(fmaxf(acc[p][r] + bias[m], 0.0f)));
+                w[i] = relu_pk(pack2(acc[p][r] + bias[m], acc[p][r + 1] + bias[m + 1]));
             }
             if (!LAST) {
 #pragma unroll
                 for (int q = 0; q < 2; ++q)
-                    Bn[p][2 * t + q] = __builtin_bit_cast(
-                        u16x8, make_uint4(pack2(v[8 * q], v[8 * q + 1]), pack2(v[8 * q + 2], v[8 * q + 3]),
-                                          pack2(v[8 * q + 4], v[8 * q + 5]), pack2(v[8 * q + 6], v[8 * q + 7])));
+                    Bn[p][2 * t + q] =
+                        __builtin_bit_cast(u16x8, make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]));
             } else {
 #pragma unroll
-                for (int r = 0; r < 16; ++r) part[p] = fmaf(w4[32 * t + 8 * (r >> 2) + 4 * h + (r & 3)], v[r], part[p]);
+                for (int i = 0; i < 8; ++i) {
+                    const int r = 2 * i;
+                    const int m = 32 * t + 8 * (r >> 2) + 4 * h + (r & 3);
+                    part[p] = fmaf(w4[m], __uint_as_float(w[i] << 16), part[p]);
+                    part[p] = fmaf(w4[m + 1], __uint_as_float(w[i] & 0xFFFF0000u), part[p]);
+                }
             }
         }
         __builtin_amdgcn_sched_barrier(0);
