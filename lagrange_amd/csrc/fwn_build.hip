@@ -73,13 +73,19 @@ __device__ __forceinline__ float wave_max(float v)
     return v;
 }
 
-__global__ __launch_bounds__(BT) void k_tri_setup(const float* __restrict__ V, const int32_t* __restrict__ F,
+// Per-triangle centroid (and box, for the SAH builder), per-mesh centroid and vertex bounds.
+// The bounds are reduced per wave, then across the workgroup's 16 waves in LDS, so a
+// single-mesh block issues 12 atomics; a wave that straddles a mesh boundary falls back to
+// per-lane atomics.
+constexpr int TS = 1024; // threads per k_tri_setup block
+__global__ __launch_bounds__(TS) void k_tri_setup(const float* __restrict__ V, const int32_t* __restrict__ F,
                                                   int64_t nF, const int64_t* __restrict__ f_off,
                                                   const int64_t* __restrict__ v_off, int M,
                                                   float* __restrict__ cen, int32_t* __restrict__ boxes,
                                                   float* __restrict__ tbox, int32_t* __restrict__ err)
 {
-    int64_t t = (int64_t)blockIdx.x * BT + threadIdx.x;
+    __shared__ int32_t part[TS / 64][13]; // per wave: mesh (-1: nothing to add), 12 bound words
+    int64_t t = (int64_t)blockIdx.x * TS + threadIdx.x;
     bool valid = t < nF;
     int m = valid ? mesh_of(f_off, M, t) : 0;
     float c[3] = {0, 0, 0}, lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -118,7 +124,9 @@ __global__ __launch_bounds__(BT) void k_tri_setup(const float* __restrict__ V, c
         clo[d] = valid ? c[d] : INFINITY;
         chi[d] = valid ? c[d] : -INFINITY;
     }
+    const int w = threadIdx.x >> 6;
     int m0 = __builtin_amdgcn_readfirstlane(m);
+    const bool any = __ballot(valid) != 0ull;
     bool same = __ballot(valid && m != m0) == 0ull;
     if (same) {
         for (int d = 0; d < 3; ++d) {
@@ -127,16 +135,45 @@ __global__ __launch_bounds__(BT) void k_tri_setup(const float* __restrict__ V, c
             lo[d] = wave_min(lo[d]);
             hi[d] = wave_max(hi[d]);
         }
-        if ((threadIdx.x & 63) == 0 && __ballot(valid) != 0ull) {
-            int32_t* b = boxes + 12 * m0;
+        if ((threadIdx.x & 63) == 0) {
+            part[w][0] = any ? m0 : -1;
             for (int d = 0; d < 3; ++d) {
-                atomicMin(b + d, f2ord(clo[d]));
-                atomicMax(b + 3 + d, f2ord(chi[d]));
-                atomicMin(b + 6 + d, f2ord(lo[d]));
-                atomicMax(b + 9 + d, f2ord(hi[d]));
+                part[w][1 + d] = f2ord(clo[d]);
+                part[w][4 + d] = f2ord(chi[d]);
+                part[w][7 + d] = f2ord(lo[d]);
+                part[w][10 + d] = f2ord(hi[d]);
             }
         }
-    } else if (valid) {
+    } else {
+        if ((threadIdx.x & 63) == 0) part[w][0] = -1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 12) { // word k of the bounds: min for k % 6 < 3, max otherwise
+        const int k = threadIdx.x;
+        const bool is_min = (k % 6) < 3;
+        int cur = -1;
+        int32_t acc = 0;
+        for (int ww = 0; ww < TS / 64; ++ww) {
+            const int mw = part[ww][0];
+            if (mw < 0) continue;
+            const int32_t v = part[ww][1 + k];
+            if (mw != cur) {
+                if (cur >= 0) {
+                    if (is_min) atomicMin(boxes + 12 * cur + k, acc);
+                    else atomicMax(boxes + 12 * cur + k, acc);
+                }
+                cur = mw;
+                acc = v;
+            } else {
+                acc = is_min ? min(acc, v) : max(acc, v);
+            }
+        }
+        if (cur >= 0) {
+            if (is_min) atomicMin(boxes + 12 * cur + k, acc);
+            else atomicMax(boxes + 12 * cur + k, acc);
+        }
+    }
+    if (!same && valid) {
         int32_t* b = boxes + 12 * m;
         for (int d = 0; d < 3; ++d) {
             atomicMin(b + d, f2ord(clo[d]));
@@ -686,7 +723,7 @@ void build_engine(Engine& e, int32_t M, const float* V_host, const int64_t* v_of
     cen.alloc(sizeof(float) * 3 * (size_t)std::max<int64_t>(nF, 1));
     if (sah) tbox.alloc(sizeof(float) * 6 * (size_t)std::max<int64_t>(nF, 1));
     if (nF) {
-        hipLaunchKernelGGL(k_tri_setup, dim3(grid_blocks(nF, BT)), dim3(BT), 0, s, dV.as<float>(), dF.as<int32_t>(),
+        hipLaunchKernelGGL(k_tri_setup, dim3(grid_blocks(nF, TS)), dim3(TS), 0, s, dV.as<float>(), dF.as<int32_t>(),
                            nF, dfo.as<int64_t>(), dvo.as<int64_t>(), M, cen.as<float>(), boxes.as<int32_t>(),
                            sah ? tbox.as<float>() : nullptr, err.as<int32_t>());
         FWN_LAUNCH_CHECK("k_tri_setup");

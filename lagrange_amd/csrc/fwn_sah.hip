@@ -39,7 +39,7 @@ namespace fwn {
 namespace {
 
 constexpr int kSahBins = 16;
-constexpr int kSahSmall = 256;  // segments up to this size are finished by one wave in LDS
+constexpr int kSahSmall = 128;  // segments up to this size are finished by one wave in LDS
 constexpr int kSahChunk = 2048; // triangles per workgroup in the level-synchronous phase
 constexpr int kSahThreads = 256;
 // per-segment accumulator (ints): centroid lo[3] hi[3] (order-preserving), bin counts[16],
