@@ -138,7 +138,7 @@ def test_sah_buckets_and_multi_mesh(oracle):
 
 
 def test_sah_large_mesh_matches_restated_order(oracle):
-    """The level-synchronous phase (segments above 256 triangles, several chunks per segment)
+    """The level-synchronous phase (segments above 128 triangles, several chunks per segment)
     on a 200k-triangle mesh: same triangle order and tree as the restatement."""
     V, F = W.displaced_sphere(100, 4, amplitude=0.15, delete_fraction=0.05).f32()
     e = FastWindingNumber((V, F), **SAH)
