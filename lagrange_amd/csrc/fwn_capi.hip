@@ -58,6 +58,67 @@ Engine* make_engine(const fwn_options* opt)
     return e;
 }
 
+} // namespace
+
+// Single-point query (the reference's per-point solid_angle / is_inside).  Concurrent callers
+// on one engine are coalesced: each request joins the pending list; a caller that finds no
+// batch running takes the whole list, answers it with one batched query (results do not depend
+// on which points share a batch) and wakes the others.  Throughput therefore grows with the
+// number of calling threads instead of being one GPU round trip per point.
+void point_query(Engine& e, const float q[3], float* omega, uint8_t* inside)
+{
+    PointRequest r;
+    std::memcpy(r.q, q, sizeof(r.q));
+    std::unique_lock<std::mutex> lk(e.pt_mu);
+    e.pt_pending.push_back(&r);
+    while (!r.done) {
+        if (e.pt_busy) {
+            e.pt_cv.wait(lk);
+            continue;
+        }
+        e.pt_busy = true;
+        std::vector<PointRequest*> batch;
+        batch.swap(e.pt_pending);
+        lk.unlock();
+        const size_t n = batch.size();
+        int st = FWN_OK;
+        std::string msg;
+        std::vector<float> Q(3 * n), om(n, 0.0f);
+        std::vector<uint8_t> in(n, 0);
+        try {
+            for (size_t k = 0; k < n; ++k) std::memcpy(&Q[3 * k], batch[k]->q, sizeof(batch[k]->q));
+            std::lock_guard<std::mutex> g(e.mu);
+            DeviceGuard dg(e.device);
+            run_queries(e, Q.data(), nullptr, (int64_t)n, om.data(), in.data(), nullptr, false);
+        } catch (const Failure& f) {
+            st = f.status;
+            msg = f.msg;
+        } catch (const std::bad_alloc&) {
+            st = FWN_OUT_OF_MEMORY;
+            msg = "host allocation failed";
+        } catch (...) {
+            st = FWN_HIP_ERROR;
+            msg = "unknown failure";
+        }
+        lk.lock();
+        for (size_t k = 0; k < n; ++k) {
+            batch[k]->omega = om[k];
+            batch[k]->inside = in[k];
+            batch[k]->status = st;
+            batch[k]->err = msg;
+            batch[k]->done = true;
+        }
+        e.pt_busy = false;
+        e.pt_cv.notify_all();
+    }
+    lk.unlock();
+    if (r.status != FWN_OK) throw Failure{r.status, r.err};
+    *omega = r.omega;
+    *inside = r.inside;
+}
+
+namespace {
+
 void destroy_engine(Engine* e)
 {
     if (!e) return;
@@ -296,9 +357,8 @@ int fwn_solid_angle_point(fwn_engine* h, const float q[3], float* omega_out)
         fwn::Engine* e = E(h);
         if (!q || !omega_out) fwn::fail(FWN_INVALID_ARGUMENT, "NULL point or output");
         if (e->n_meshes != 1) fwn::fail(FWN_INVALID_ARGUMENT, "multi-mesh engine has no single-point query");
-        std::lock_guard<std::mutex> lk(e->mu);
-        fwn::DeviceGuard g(e->device);
-        fwn::run_queries(*e, q, nullptr, 1, omega_out, nullptr, nullptr, false);
+        uint8_t inside = 0;
+        fwn::point_query(*e, q, omega_out, &inside);
     });
 }
 
@@ -308,10 +368,8 @@ int fwn_is_inside_point(fwn_engine* h, const float q[3], uint8_t* inside_out)
         fwn::Engine* e = E(h);
         if (!q || !inside_out) fwn::fail(FWN_INVALID_ARGUMENT, "NULL point or output");
         if (e->n_meshes != 1) fwn::fail(FWN_INVALID_ARGUMENT, "multi-mesh engine has no single-point query");
-        std::lock_guard<std::mutex> lk(e->mu);
-        fwn::DeviceGuard g(e->device);
         float om = 0.0f;
-        fwn::run_queries(*e, q, nullptr, 1, &om, inside_out, nullptr, false);
+        fwn::point_query(*e, q, &om, inside_out);
     });
 }
 

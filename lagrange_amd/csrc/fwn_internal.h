@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <condition_variable>
 #include <cstdint>
 #include <mutex>
 #include <new>
@@ -57,10 +58,26 @@ struct DBuf {
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
+// one single-point query waiting to be answered in a coalesced batch
+struct PointRequest {
+    float q[3];
+    float omega = 0.0f;
+    uint8_t inside = 0;
+    int status = 0;
+    std::string err;
+    bool done = false;
+};
+
 struct Engine {
     int device = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;
+    // concurrent single-point calls are answered together: a caller that finds no batch running
+    // takes every pending request and answers them in one query
+    std::mutex pt_mu;
+    std::condition_variable pt_cv;
+    std::vector<PointRequest*> pt_pending;
+    bool pt_busy = false;
     fwn_options opt{};
 
     int32_t n_meshes = 0;

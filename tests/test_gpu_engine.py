@@ -391,3 +391,19 @@ def test_small_calls_equal_sorted_calls(ico5):
     np.testing.assert_array_equal(om_m[300:], om_big[:500])
     np.testing.assert_array_equal(om_m[:300], FastWindingNumber(meshes[0]).solid_angle_batch(
         np.concatenate([pts[0], rng.uniform(-1.3, 1.3, (5000, 3)).astype(np.float32)]))[:300])
+
+
+def test_concurrent_point_calls_are_coalesced(ico5):
+    """Per-point calls from many threads on one engine (the reference's calling pattern from TBB
+    workers) are answered in shared batches: every thread gets the value of a batched call."""
+    from concurrent.futures import ThreadPoolExecutor
+    V, F = ico5
+    e = FastWindingNumber((V, F))
+    rng = np.random.default_rng(5)
+    P = rng.uniform(-1.5, 1.5, (3000, 3)).astype(np.float32)
+    ref, ins_ref = e.solid_angle_batch(P, with_inside=True)
+    with ThreadPoolExecutor(16) as ex:
+        om = list(ex.map(e.solid_angle, P))
+        ins = list(ex.map(e.is_inside, P[:500]))
+    np.testing.assert_array_equal(np.array(om, np.float32), ref)
+    np.testing.assert_array_equal(np.array(ins), ins_ref[:500])
