@@ -124,9 +124,16 @@ void destroy_engine(Engine* e)
     if (!e) return;
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->cs_in) (void)hipStreamSynchronize(e->cs_in);
+    if (e->cs_out) (void)hipStreamSynchronize(e->cs_out);
     for (auto& ev : e->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (e->h_stage) (void)hipHostFree(e->h_stage);
+    if (e->h_pipe) (void)hipHostFree(e->h_pipe);
+    for (auto& ev : e->pev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (e->cs_in) (void)hipStreamDestroy(e->cs_in);
+    if (e->cs_out) (void)hipStreamDestroy(e->cs_out);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }

@@ -115,6 +115,13 @@ struct Engine {
     void* h_stage = nullptr;
     size_t h_stage_bytes = 0;
     DBuf q_meta; // device copy of the staged metadata (and small host batches' points)
+    // pipelined host calls: pinned double buffers (points in, results out), a stream per copy
+    // direction, per-chunk flags
+    void* h_pipe = nullptr;
+    size_t h_pipe_bytes = 0;
+    hipStream_t cs_in = nullptr, cs_out = nullptr;
+    hipEvent_t pev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}; // h2d[2], comp[2], d2h[2]
+    DBuf q_pmeta, q_pflags;
 
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double last_traverse_ms = 0.0;

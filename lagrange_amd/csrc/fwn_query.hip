@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <thread>
 
 #include <algorithm>
 #include <cmath>
@@ -480,6 +481,172 @@ char* host_stage(Engine& e, size_t bytes)
     return static_cast<char*>(e.h_stage);
 }
 
+// Host calls of a single-mesh engine larger than one chunk are pipelined: chunk k's points are
+// copied (several host threads) into a pinned buffer and sent to the device on one copy stream
+// while chunk k-1 is answered on the engine's stream and chunk k-2's results come back on a
+// second copy stream and are copied out to the caller's arrays.  Each chunk is an independent
+// Morton-sorted query (a point's result does not depend on the other points of its call).
+constexpr int64_t kPipeChunk = 1 << 21; // points per chunk
+
+void parallel_copy(void* dst, const void* src, size_t bytes)
+{
+    const size_t piece = (size_t)4 << 20; // at least 4 MB per thread
+    const int nt = (int)std::min<size_t>(8, std::max<size_t>(1, bytes / piece));
+    if (nt <= 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t part = (bytes + nt - 1) / nt;
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) {
+        const size_t a = (size_t)t * part, b = std::min(bytes, a + part);
+        if (a < b) th.emplace_back([=] { std::memcpy((char*)dst + a, (const char*)src + a, b - a); });
+    }
+    std::memcpy(dst, src, std::min(bytes, part));
+    for (auto& x : th) x.join();
+}
+
+void run_queries_pipelined(Engine& e, const float* Q, int64_t nQ, float* omega, uint8_t* inside, int banks)
+{
+    hipStream_t s = e.stream;
+    const int64_t C = kPipeChunk;
+    const int64_t nch = (nQ + C - 1) / C;
+    const size_t in_b = sizeof(float) * 3 * (size_t)C, om_b = sizeof(float) * (size_t)C, i8_b = (size_t)C;
+    const size_t slot_b = in_b + om_b + i8_b;
+    if (e.h_pipe_bytes < 2 * slot_b) {
+        if (e.h_pipe) FWN_HIP(hipHostFree(e.h_pipe));
+        e.h_pipe = nullptr;
+        e.h_pipe_bytes = 0;
+        FWN_HIP(hipHostMalloc(&e.h_pipe, 2 * slot_b, hipHostMallocDefault));
+        e.h_pipe_bytes = 2 * slot_b;
+    }
+    if (!e.cs_in) FWN_HIP(hipStreamCreateWithFlags(&e.cs_in, hipStreamNonBlocking));
+    if (!e.cs_out) FWN_HIP(hipStreamCreateWithFlags(&e.cs_out, hipStreamNonBlocking));
+    if (!e.pev[0])
+        for (auto& ev : e.pev) FWN_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    char* pin = static_cast<char*>(e.h_pipe);
+    auto pin_in = [&](int sl) { return reinterpret_cast<float*>(pin + sl * slot_b); };
+    auto pin_om = [&](int sl) { return reinterpret_cast<float*>(pin + sl * slot_b + in_b); };
+    auto pin_i8 = [&](int sl) { return reinterpret_cast<uint8_t*>(pin + sl * slot_b + in_b + om_b); };
+
+    e.q_in.reserve(sizeof(float) * 3 * nQ);
+    e.q_om.reserve(sizeof(float) * nQ);
+    if (inside) e.q_in8.reserve(nQ);
+    float* dIn3 = e.q_in.as<float>();
+    float* dOm = e.q_om.as<float>();
+    uint8_t* dIn = inside ? e.q_in8.as<uint8_t>() : nullptr;
+    for (int k = 0; k < 2; ++k) {
+        e.q_keys[k].reserve(sizeof(uint32_t) * C);
+        e.q_vals[k].reserve(sizeof(uint32_t) * C);
+    }
+    if (e.acc_cap < C) {
+        e.q_acc.alloc(sizeof(unsigned long long) * C);
+        e.acc_cap = C;
+        hipLaunchKernelGGL(k_fill_u64, dim3(grid_blocks(C, 256)), dim3(256), 0, s, e.q_acc.as<unsigned long long>(),
+                           C);
+        FWN_LAUNCH_CHECK("k_fill_u64");
+    }
+    if (e.item_cap < std::max<int64_t>(C, 1 << 20)) {
+        e.item_cap = std::max<int64_t>(C, 1 << 20);
+        e.q_items.alloc(sizeof(uint32_t) * 3 * (size_t)e.item_cap);
+    }
+    // per chunk: q_off {0, n}, b_off {0, batches}, mesh order {0} (5 int64 words)
+    std::vector<int64_t> meta(5 * nch, 0);
+    for (int64_t k = 0; k < nch; ++k) {
+        const int64_t ck = std::min(C, nQ - k * C);
+        meta[5 * k + 1] = ck;
+        meta[5 * k + 3] = (ck + 63) / 64;
+    }
+    e.q_pmeta.reserve(sizeof(int64_t) * meta.size());
+    FWN_HIP(hipMemcpyAsync(e.q_pmeta.p, meta.data(), sizeof(int64_t) * meta.size(), hipMemcpyHostToDevice, s));
+    e.q_pflags.reserve(sizeof(int32_t) * 4 * nch);
+    const int waves = e.opt.waves_per_block > 0 ? e.opt.waves_per_block : 1;
+    const bool persist = e.opt.launch_mode == FWN_LAUNCH_PERSISTENT;
+    if (persist && e.n_cu == 0) {
+        hipDeviceProp_t prop;
+        FWN_HIP(hipGetDeviceProperties(&prop, e.device));
+        e.n_cu = prop.multiProcessorCount;
+    }
+    const float beta2 = e.opt.accuracy_scale * e.opt.accuracy_scale;
+    const uint32_t cap = (uint32_t)e.item_cap;
+
+    for (int64_t k = 0; k <= nch; ++k) {
+        if (k < nch) {
+            const int sl = (int)(k & 1);
+            const int64_t off = k * C, ck = std::min(C, nQ - off);
+            if (k >= 2) FWN_HIP(hipEventSynchronize(e.pev[sl])); // chunk k-2's upload has read the buffer
+            parallel_copy(pin_in(sl), Q + 3 * off, sizeof(float) * 3 * (size_t)ck);
+            FWN_HIP(hipMemcpyAsync(dIn3 + 3 * off, pin_in(sl), sizeof(float) * 3 * (size_t)ck, hipMemcpyHostToDevice,
+                                   e.cs_in));
+            FWN_HIP(hipEventRecord(e.pev[sl], e.cs_in));
+            FWN_HIP(hipStreamWaitEvent(s, e.pev[sl], 0));
+            // chunk k on the engine's stream: sort, traversal, FP64 pass (no host wait)
+            const float* dQ = dIn3 + 3 * off;
+            const int64_t* d_qoff = e.q_pmeta.as<int64_t>() + 5 * k;
+            const int64_t* d_boff = d_qoff + 2;
+            const int32_t* d_order = reinterpret_cast<const int32_t*>(d_qoff + 4);
+            uint32_t* kp = e.q_keys[0].as<uint32_t>();
+            uint32_t* vp = e.q_vals[0].as<uint32_t>();
+            uint32_t* kp2 = e.q_keys[1].as<uint32_t>();
+            uint32_t* vp2 = e.q_vals[1].as<uint32_t>();
+            hipLaunchKernelGGL(k_query_keys, dim3(grid_blocks(ck, QT)), dim3(QT), 0, s, dQ, ck, d_qoff, 1, 0,
+                               e.mesh_info.as<float>(), kp, vp);
+            FWN_LAUNCH_CHECK("k_query_keys");
+            radix_sort_pairs32(kp, vp, kp2, vp2, ck, 8, 32, e.q_hist, e.q_rowsum, s);
+            const int64_t nb = (ck + 63) / 64;
+            int64_t nblk = (nb + waves - 1) / waves;
+            if (persist) nblk = std::min<int64_t>(nblk, (int64_t)e.n_cu * resident_blocks(true, waves, banks));
+            int32_t* dflag = e.q_pflags.as<int32_t>() + 4 * k;
+            uint32_t* dnitems = reinterpret_cast<uint32_t*>(dflag + 1);
+            uint32_t* dwork = reinterpret_cast<uint32_t*>(dflag + 2);
+            hipLaunchKernelGGL(k_zero3, dim3(1), dim3(1), 0, s, dflag);
+            launch_traverse(false, waves, persist, banks, (unsigned)nblk, s, e.slots.as<uint32_t>(), e.width,
+                            e.tris.as<uint32_t>(), e.roots.as<int32_t>(), e.mesh_info.as<float>(), dQ, vp, d_qoff,
+                            d_boff, d_order, 1, nb, 64, beta2, dOm + off, dIn ? dIn + off : nullptr, (int32_t*)nullptr,
+                            e.q_items.as<uint32_t>(), dnitems, cap, dflag, dwork);
+            FWN_LAUNCH_CHECK("k_traverse");
+            hipLaunchKernelGGL(k_near64, dim3(2048), dim3(256), 0, s, e.q_items.as<uint32_t>(), dnitems, cap,
+                               e.tris.as<uint32_t>(), dQ, vp, e.q_acc.as<unsigned long long>());
+            FWN_LAUNCH_CHECK("k_near64");
+            hipLaunchKernelGGL(k_near64_combine, dim3(2048), dim3(256), 0, s, e.q_items.as<uint32_t>(), dnitems,
+                               cap, vp, e.q_acc.as<unsigned long long>(), dOm + off, dIn ? dIn + off : nullptr);
+            FWN_LAUNCH_CHECK("k_near64_combine");
+            FWN_HIP(hipEventRecord(e.pev[2 + sl], s));
+            // chunk k's results back to the pinned buffer on the other copy stream; chunk k-2's
+            // results were copied out of that buffer in the previous iteration
+            FWN_HIP(hipStreamWaitEvent(e.cs_out, e.pev[2 + sl], 0));
+            FWN_HIP(hipMemcpyAsync(pin_om(sl), dOm + off, sizeof(float) * (size_t)ck, hipMemcpyDeviceToHost, e.cs_out));
+            if (inside)
+                FWN_HIP(hipMemcpyAsync(pin_i8(sl), dIn + off, (size_t)ck, hipMemcpyDeviceToHost, e.cs_out));
+            FWN_HIP(hipEventRecord(e.pev[4 + sl], e.cs_out));
+        }
+        if (k >= 1) { // chunk k-1 to the caller's arrays
+            const int64_t j = k - 1;
+            const int sl = (int)(j & 1);
+            const int64_t off = j * C, cj = std::min(C, nQ - off);
+            FWN_HIP(hipEventSynchronize(e.pev[4 + sl]));
+            parallel_copy(omega + off, pin_om(sl), sizeof(float) * (size_t)cj);
+            if (inside) parallel_copy(inside + off, pin_i8(sl), (size_t)cj);
+        }
+    }
+    std::vector<int32_t> fl(4 * nch);
+    FWN_HIP(hipMemcpyAsync(fl.data(), e.q_pflags.p, sizeof(int32_t) * 4 * nch, hipMemcpyDeviceToHost, s));
+    FWN_HIP(hipStreamSynchronize(s));
+    FWN_HIP(hipStreamSynchronize(e.cs_in));
+    FWN_HIP(hipStreamSynchronize(e.cs_out));
+    for (int64_t k = 0; k < nch; ++k) {
+        if (fl[4 * k]) fail(FWN_LIMIT_EXCEEDED, "traversal stack overflow");
+        const int64_t need = (int64_t)(uint32_t)fl[4 * k + 1] * kItemChunk;
+        if (need > (int64_t)cap) { // the chunk's FP64 work list overflowed its capacity in the
+                                   // pipeline (a redo may have grown it since): answer it again
+            const int64_t off = k * C, ck = std::min(C, nQ - off);
+            run_queries(e, Q + 3 * off, nullptr, ck, omega + off, inside ? inside + off : nullptr, nullptr, false);
+        }
+    }
+    e.last_traverse_ms = 0.0;
+    e.last_total_ms = 0.0;
+}
+
 } // namespace
 
 void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t nQ, float* omega, uint8_t* inside,
@@ -504,6 +671,10 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
     if (nQ > (int64_t)UINT32_MAX / 2) fail(FWN_LIMIT_EXCEEDED, "more than 2^31 points in one call");
     const int banks = stack_banks(e.width, e.max_depth);
     if (banks == 0) fail(FWN_LIMIT_EXCEEDED, "tree deeper than the traversal stack supports");
+    if (!dev && !stats && M == 1 && nQ > kPipeChunk) {
+        run_queries_pipelined(e, Q, nQ, omega, inside, banks);
+        return;
+    }
 
     // batches of 64 points, numbered over the meshes in decreasing triangle count so that a
     // persistent launch starts the deepest trees first

@@ -407,3 +407,19 @@ def test_concurrent_point_calls_are_coalesced(ico5):
         ins = list(ex.map(e.is_inside, P[:500]))
     np.testing.assert_array_equal(np.array(om, np.float32), ref)
     np.testing.assert_array_equal(np.array(ins), ins_ref[:500])
+
+
+def test_pipelined_host_calls_equal_device_calls():
+    """Host-array calls larger than one chunk (2,097,152 points) go through the pinned
+    double-buffered pipeline (chunks sorted and answered separately): every value and flag equals
+    the same call on device arrays, and calls just above and at the chunk size agree as well."""
+    import torch
+    V, F = W.icosphere(24).f32()
+    e = FastWindingNumber((V, F))
+    rng = np.random.default_rng(8)
+    for n in (2_097_152, 2_097_153, 5_000_000):
+        q = rng.uniform(-1.4, 1.4, (n, 3)).astype(np.float32)
+        om_h, in_h = e.solid_angle_batch(q, with_inside=True)
+        om_d, in_d = e.solid_angle_batch(torch.from_numpy(q).cuda(), with_inside=True)
+        np.testing.assert_array_equal(om_h, om_d.cpu().numpy())
+        np.testing.assert_array_equal(in_h, in_d.cpu().numpy())
