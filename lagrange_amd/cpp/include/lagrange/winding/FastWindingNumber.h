@@ -18,6 +18,20 @@ namespace lagrange {
 namespace winding {
 
 ///
+/// Engine options (an addition; the reference engine always uses its defaults, which these are).
+///
+struct FastWindingNumberOptions
+{
+    /// beta of the far-field criterion |q - p|^2 > beta^2 r^2 (reference default 2; 2.75 keeps the
+    /// winding number within 1e-3 of the exact sum on the benchmark meshes)
+    float accuracy_scale = 2.0f;
+    /// triangles per leaf bucket, 1..8 (1: the reference's single-triangle items)
+    int leaf_max = 1;
+    /// true: the reference's box-area tree (top-down SAH); false: Morton LBVH
+    bool sah_tree = true;
+};
+
+///
 /// Fast winding number computation for triangle soups.
 ///
 class FastWindingNumber
@@ -30,6 +44,10 @@ public:
     ///
     template <typename Scalar, typename Index>
     FastWindingNumber(const SurfaceMesh<Scalar, Index>& mesh);
+
+    /// Same with explicit engine options (an addition to the reference API).
+    template <typename Scalar, typename Index>
+    FastWindingNumber(const SurfaceMesh<Scalar, Index>& mesh, const FastWindingNumberOptions& options);
 
     FastWindingNumber();
     ~FastWindingNumber();

@@ -34,7 +34,7 @@ struct FastWindingNumber::Impl
     }
 
     template <typename VView, typename FView>
-    void initialize(const VView& vertices, const FView& facets)
+    void initialize(const VView& vertices, const FView& facets, const fwn_options* opt = nullptr)
     {
         if (vertices.cols() != 3) throw Error("Fast winding number engine only supports 3D meshes");
         if (facets.cols() != 3) throw Error("Fast winding number engine only supports triangle meshes");
@@ -44,7 +44,7 @@ struct FastWindingNumber::Impl
         std::vector<int32_t> f(static_cast<size_t>(facets.rows()) * 3);
         for (std::ptrdiff_t i = 0; i < (std::ptrdiff_t)facets.rows(); ++i)
             for (int d = 0; d < 3; ++d) f[3 * i + d] = static_cast<int32_t>(facets(i, d));
-        int st = fwn_create(&engine, v.data(), (int64_t)vertices.rows(), f.data(), (int64_t)facets.rows(), nullptr);
+        int st = fwn_create(&engine, v.data(), (int64_t)vertices.rows(), f.data(), (int64_t)facets.rows(), opt);
         if (st != FWN_OK) raise("fwn_create", st);
     }
 
@@ -62,6 +62,20 @@ FastWindingNumber::FastWindingNumber(const SurfaceMesh<Scalar, Index>& mesh) : m
     if (mesh.get_dimension() != 3) throw Error("Fast winding number engine only supports 3D meshes");
     if (!mesh.is_triangle_mesh()) throw Error("Fast winding number engine only supports triangle meshes");
     m_impl->initialize(vertex_view(mesh), facet_view(mesh));
+}
+
+template <typename Scalar, typename Index>
+FastWindingNumber::FastWindingNumber(const SurfaceMesh<Scalar, Index>& mesh, const FastWindingNumberOptions& options)
+    : m_impl(make_value_ptr<Impl>())
+{
+    if (mesh.get_dimension() != 3) throw Error("Fast winding number engine only supports 3D meshes");
+    if (!mesh.is_triangle_mesh()) throw Error("Fast winding number engine only supports triangle meshes");
+    fwn_options o;
+    fwn_options_default(&o);
+    o.accuracy_scale = options.accuracy_scale;
+    o.leaf_max = options.leaf_max;
+    o.tree_builder = options.sah_tree ? FWN_TREE_SAH : FWN_TREE_LBVH;
+    m_impl->initialize(vertex_view(mesh), facet_view(mesh), &o);
 }
 
 FastWindingNumber::FastWindingNumber() = default;
@@ -155,6 +169,10 @@ template FastWindingNumber::FastWindingNumber(const SurfaceMesh<float, uint32_t>
 template FastWindingNumber::FastWindingNumber(const SurfaceMesh<double, uint32_t>&);
 template FastWindingNumber::FastWindingNumber(const SurfaceMesh<float, uint64_t>&);
 template FastWindingNumber::FastWindingNumber(const SurfaceMesh<double, uint64_t>&);
+template FastWindingNumber::FastWindingNumber(const SurfaceMesh<float, uint32_t>&, const FastWindingNumberOptions&);
+template FastWindingNumber::FastWindingNumber(const SurfaceMesh<double, uint32_t>&, const FastWindingNumberOptions&);
+template FastWindingNumber::FastWindingNumber(const SurfaceMesh<float, uint64_t>&, const FastWindingNumberOptions&);
+template FastWindingNumber::FastWindingNumber(const SurfaceMesh<double, uint64_t>&, const FastWindingNumberOptions&);
 
 } // namespace winding
 } // namespace lagrange

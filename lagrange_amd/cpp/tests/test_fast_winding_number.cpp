@@ -4,6 +4,7 @@
 // (modules/conftest.py:29-37, 108-144) and a subdivided octahedron projected to the sphere.
 #include <lagrange/winding/FastWindingNumber.h>
 
+#include <algorithm>
 #include <array>
 #include <chrono>
 #include <cmath>
@@ -219,6 +220,35 @@ int main()
                  CHECK(f[i] >= 0 && f[i] < 12);
              }
              CHECK(std::fabs(cp[0][2] - 1.0) < 1e-12 && std::fabs(cp[2][0] - 1.0) < 1e-12);
+         }},
+        {"engine options: accuracy scale, leaf buckets, Morton tree", [] {
+             auto m = sphere(24);
+             FastWindingNumber def(m), acc(m, {2.75f, 1, true}), fast(m, {2.0f, 8, false});
+             // closed sphere: every variant is exact far from the surface and inside
+             for (float z : {-0.5f, 0.0f, 0.5f}) {
+                 CHECK(def.is_inside({0.1f, 0.2f, z}) && acc.is_inside({0.1f, 0.2f, z}) && fast.is_inside({0.1f, 0.2f, z}));
+                 CHECK(!def.is_inside({1.5f, 0.2f, z}) && !acc.is_inside({1.5f, 0.2f, z}) && !fast.is_inside({1.5f, 0.2f, z}));
+             }
+             // near the surface (inside a closed sphere: exactly 4 pi) every variant stays within the
+             // order-2 truncation bound, and the larger accuracy scale within 1e-3 in w
+             double e_def = 0.0, e_acc = 0.0, e_fast = 0.0;
+             for (int i = 0; i < 64; ++i) {
+                 const float t = 0.1f * i, r = 0.97f;
+                 const std::array<float, 3> p = {r * std::sin(t) * std::cos(2.3f * t), r * std::sin(t) * std::sin(2.3f * t),
+                                                 r * std::cos(t)};
+                 e_def = std::max(e_def, std::fabs(def.solid_angle(p) - 4 * kPi) / (4 * kPi));
+                 e_acc = std::max(e_acc, std::fabs(acc.solid_angle(p) - 4 * kPi) / (4 * kPi));
+                 e_fast = std::max(e_fast, std::fabs(fast.solid_angle(p) - 4 * kPi) / (4 * kPi));
+             }
+             std::printf("  max |dw| at 0.97 R: beta 2 %.2e, beta 2.75 %.2e, leaf 8 Morton %.2e\n", e_def, e_acc, e_fast);
+             CHECK(e_def <= 4e-3 && e_fast <= 4e-3 && e_acc <= 1e-3);
+             bool threw = false;
+             try {
+                 FastWindingNumber bad(m, {2.0f, 9, true});
+             } catch (const lagrange::Error&) {
+                 threw = true;
+             }
+             CHECK(threw);
          }},
         {"concurrent const queries", [] {
              FastWindingNumber e(sphere(16));
