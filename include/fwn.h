@@ -164,7 +164,9 @@ int fwn_destroy(fwn_engine* e);
 /* ---- queries -------------------------------------------------------------------------- */
 /* Solid angle (steradians, as FastWindingNumber::solid_angle) of nQ points Q (nQ x 3 float).
  * inside_out (nullable) receives (omega / (4 pi) > 0.5) evaluated in double, byte per point
- * (FastWindingNumber.cpp:66).  flags: FWN_DEVICE_POINTERS. */
+ * (FastWindingNumber.cpp:66).  flags: FWN_DEVICE_POINTERS.  Results do not depend on how the
+ * points are batched: host calls above 2^21 points are answered in pipelined chunks, calls of at
+ * most 2,048 points without the Morton sort. */
 int fwn_solid_angle(fwn_engine* e, const float* Q, int64_t nQ, float* omega_out, uint8_t* inside_out,
                     int32_t flags);
 
@@ -216,7 +218,10 @@ int fwn_sample_inside(fwn_engine* e, const float lo[3], const float hi[3], int64
 int fwn_closest_point(fwn_engine* e, const float* Q, int64_t nQ, double* sq_dist_out, int32_t* facet_out,
                       double* closest_out, int32_t flags);
 
-/* Single point, host memory, as the reference's per-point API.  Runs one GPU launch. */
+/* Single point, host memory, as the reference's per-point API.  Concurrent calls on one engine
+ * (callers on several threads, as the reference is driven from TBB workers) are answered
+ * together: a caller that finds no batch running answers every pending point in one batched
+ * query and wakes the others; each gets exactly the value of a batched call. */
 int fwn_solid_angle_point(fwn_engine* e, const float q[3], float* omega_out);
 int fwn_is_inside_point(fwn_engine* e, const float q[3], uint8_t* inside_out);
 
