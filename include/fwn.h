@@ -107,10 +107,14 @@ enum { FWN_LAUNCH_GRID = 0, FWN_LAUNCH_PERSISTENT = 1 };
  *   3    r2    squared radius bound: every point of the slot's triangles is within sqrt(r2) of p
  *   4-6  N     sum of area-weighted normals
  *   7    tr    trace of the first moment  M1_ji = sum_t A_t (c_t-p)_j n_t,i
- *   8-13 -3 * (M1xx, M1yy, M1zz, M1xy+M1yx, M1yz+M1zy, M1zx+M1xz)
+ *   8-13 -3 * (M1xx, M1yy, M1zz, M1xy+M1yx, M1yz+M1zy, M1zx+M1xz), plus tr on xx, yy, zz
  *   14-23 7.5 * (S_xxx, S_yyy, S_zzz, 3S_xxy, 3S_xxz, 3S_yyx, 3S_yyz, 3S_zzx, 3S_zzy, 6S_xyz)
- *        where S is the fully symmetrised second moment  int (x-p)_j (x-p)_k n_i dA
- *   24-26 -4.5 * t,  t_k = sum_i S_iik
+ *        where S is the fully symmetrised second moment  int (x-p)_j (x-p)_k n_i dA,
+ *        plus C_x on xxx, yyx, zzx, C_y on yyy, xxy, zzy and C_z on zzz, xxz, yyz
+ *   24-26 C = -4.5 * t,  t_k = sum_i S_iik
+ * With d = p - q the far field is |d|^-3 (d.N + |d|^-2 (q1 + |d|^-2 q2)), q1 the quadratic form
+ * of words 8-13 and q2 the cubic of words 14-23: tr and C.d are folded into them through
+ * |d|^2 / |d|^2, and words 7 and 24-26 are not read by the evaluation.
  *   27   child node index (internal slot) or -1 (leaf slot)
  *   28   first triangle (position in the engine's sorted triangle array)
  *   29   triangle count covered by the slot

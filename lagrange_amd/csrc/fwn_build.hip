@@ -645,20 +645,29 @@ __global__ __launch_bounds__(BT) void k_write_slots(int64_t W, int width, const 
     f[4] = (float)r[R_N];
     f[5] = (float)r[R_N + 1];
     f[6] = (float)r[R_N + 2];
+    // tr and C are folded into the homogeneous quadratic and cubic: tr = tr |d|^2 / |d|^2 adds
+    // tr to the diagonal of A, (C.d) = (C.d) |d|^2 / |d|^2 adds C_i to the cubic monomials
+    // d_i d_j d_j; the far field then needs no constant terms (words 7 and 24-26 keep tr and C)
     const double* M = r + R_M1;
-    f[7] = (float)(M[0] + M[4] + M[8]);
-    f[8] = (float)(-3.0 * M[0]);
-    f[9] = (float)(-3.0 * M[4]);
-    f[10] = (float)(-3.0 * M[8]);
+    const double tr = M[0] + M[4] + M[8];
+    f[7] = (float)tr;
+    f[8] = (float)(-3.0 * M[0] + tr);
+    f[9] = (float)(-3.0 * M[4] + tr);
+    f[10] = (float)(-3.0 * M[8] + tr);
     f[11] = (float)(-3.0 * (M[1] + M[3]));
     f[12] = (float)(-3.0 * (M[5] + M[7]));
     f[13] = (float)(-3.0 * (M[6] + M[2]));
     const double* S = r + R_S;
     const double sc[10] = {7.5, 7.5, 7.5, 22.5, 22.5, 22.5, 22.5, 22.5, 22.5, 45.0};
-    for (int k = 0; k < 10; ++k) f[14 + k] = (float)(sc[k] * S[k]);
-    f[24] = (float)(-4.5 * (S[0] + S[5] + S[7]));
-    f[25] = (float)(-4.5 * (S[3] + S[1] + S[8]));
-    f[26] = (float)(-4.5 * (S[4] + S[6] + S[2]));
+    const double cx = -4.5 * (S[0] + S[5] + S[7]);
+    const double cy = -4.5 * (S[3] + S[1] + S[8]);
+    const double cz = -4.5 * (S[4] + S[6] + S[2]);
+    // monomials: xxx yyy zzz xxy xxz yyx yyz zzx zzy xyz
+    const double fold[10] = {cx, cy, cz, cy, cz, cx, cz, cx, cy, 0.0};
+    for (int k = 0; k < 10; ++k) f[14 + k] = (float)(sc[k] * S[k] + fold[k]);
+    f[24] = (float)cx;
+    f[25] = (float)cy;
+    f[26] = (float)cz;
     for (int k = 0; k < 27; ++k) w[k] = __float_as_uint(f[k]);
     // a leaf slot is any slot without a wide child: a single triangle or a bucket of at most
     // leaf_max triangles (an internal binary node); both cover a contiguous sorted range

@@ -129,25 +129,28 @@ __device__ __forceinline__ int32_t si(const SlotRegs& r, int k) { return (int32_
 
 // order-2 far field of one slot held in registers, added to om:
 //   omega += inv3 (o0 + inv2 (q1 + inv2 q2)),  inv = 1 / |d|,
-// o0 = d.N + tr M1, q1 = -3 d^T M1 d + C.d (C = -4.5 t), q2 = 7.5 S(d, d, d); q1 and q2 are
-// evaluated in Horner form over the packed coefficients (no monomials needed)
+// o0 = d.N, q1 = d^T A d, q2 = B(d, d, d).  The build folds the constant terms (tr M1 of o0 and
+// C.d of q1) into A and B through |d|^2 / |d|^2, so every product has one coefficient operand
+// and no coefficient needs a move into a vector register; q1 and q2 are evaluated in Horner form
+// over the packed coefficients (no monomials needed)
 __device__ __forceinline__ float far_field_acc(const SlotRegs& r, float dx, float dy, float dz, float d2, float om)
 {
     const float inv = __builtin_amdgcn_rsqf(d2);
     const float inv2 = inv * inv;
     const float inv3 = inv2 * inv;
-    const float o0 = fmaf(dx, sw(r, FWN_W_NX), fmaf(dy, sw(r, FWN_W_NY), fmaf(dz, sw(r, FWN_W_NZ), sw(r, FWN_W_TR))));
-    // A: xx yy zz xy yz zx;  C: x y z
-    const float a1 = fmaf(sw(r, FWN_W_A), dx, fmaf(sw(r, FWN_W_A + 3), dy, fmaf(sw(r, FWN_W_A + 5), dz, sw(r, FWN_W_C))));
-    const float b1 = fmaf(sw(r, FWN_W_A + 1), dy, fmaf(sw(r, FWN_W_A + 4), dz, sw(r, FWN_W_C + 1)));
-    const float c1 = fmaf(sw(r, FWN_W_A + 2), dz, sw(r, FWN_W_C + 2));
+    const float o0 = fmaf(dx, sw(r, FWN_W_NX), fmaf(dy, sw(r, FWN_W_NY), dz * sw(r, FWN_W_NZ)));
+    // A: xx yy zz xy yz zx
+    const float a1 = fmaf(sw(r, FWN_W_A), dx, fmaf(sw(r, FWN_W_A + 3), dy, sw(r, FWN_W_A + 5) * dz));
+    const float b1 = fmaf(sw(r, FWN_W_A + 1), dy, sw(r, FWN_W_A + 4) * dz);
+    const float c1 = sw(r, FWN_W_A + 2) * dz;
     const float q1 = fmaf(dx, a1, fmaf(dy, b1, dz * c1));
     // B: xxx yyy zzz xxy xxz yyx yyz zzx zzy xyz
-    const float ax = fmaf(sw(r, FWN_W_B), dx, fmaf(sw(r, FWN_W_B + 3), dy, sw(r, FWN_W_B + 4) * dz));
-    const float bx = fmaf(sw(r, FWN_W_B + 5), dy, sw(r, FWN_W_B + 9) * dz);
-    const float X = fmaf(dx, ax, fmaf(dy, bx, dz * (sw(r, FWN_W_B + 7) * dz)));
-    const float Y = fmaf(dy, fmaf(sw(r, FWN_W_B + 1), dy, sw(r, FWN_W_B + 6) * dz), dz * (sw(r, FWN_W_B + 8) * dz));
-    const float Z = dz * (dz * sw(r, FWN_W_B + 2));
+    const float zz = dz * dz;
+    const float P = fmaf(sw(r, FWN_W_B), dx, fmaf(sw(r, FWN_W_B + 3), dy, sw(r, FWN_W_B + 4) * dz));
+    const float Qv = fmaf(sw(r, FWN_W_B + 5), dy, sw(r, FWN_W_B + 9) * dz);
+    const float X = fmaf(dx, P, fmaf(dy, Qv, sw(r, FWN_W_B + 7) * zz));
+    const float Y = fmaf(dy, fmaf(sw(r, FWN_W_B + 1), dy, sw(r, FWN_W_B + 6) * dz), sw(r, FWN_W_B + 8) * zz);
+    const float Z = sw(r, FWN_W_B + 2) * zz;
     const float q2 = fmaf(dx, X, fmaf(dy, Y, dz * Z));
     const float t = fmaf(inv2, fmaf(inv2, q2, q1), o0);
     return fmaf(t, inv3, om);

@@ -277,20 +277,29 @@ static void slot_from_moments(const double* m, uint32_t* w)
     f[4] = (float)m[MN];
     f[5] = (float)m[MN + 1];
     f[6] = (float)m[MN + 2];
+    /* tr joins the diagonal of the quadratic (tr = tr |d|^2 / |d|^2) and C the cubic monomials
+     * d_i d_j d_j ((C.d) = (C.d) |d|^2 / |d|^2), so the far field has no constant terms; words
+     * 7 and 24-26 keep tr and C themselves */
     const double* M = m + M1;
-    f[7] = (float)(M[0] + M[4] + M[8]);
-    f[8] = (float)(-3.0 * M[0]);
-    f[9] = (float)(-3.0 * M[4]);
-    f[10] = (float)(-3.0 * M[8]);
+    const double tr = M[0] + M[4] + M[8];
+    f[7] = (float)tr;
+    f[8] = (float)(-3.0 * M[0] + tr);
+    f[9] = (float)(-3.0 * M[4] + tr);
+    f[10] = (float)(-3.0 * M[8] + tr);
     f[11] = (float)(-3.0 * (M[1] + M[3]));
     f[12] = (float)(-3.0 * (M[5] + M[7]));
     f[13] = (float)(-3.0 * (M[6] + M[2]));
     const double* S = m + MS;
     static const double sc[10] = {7.5, 7.5, 7.5, 22.5, 22.5, 22.5, 22.5, 22.5, 22.5, 45.0};
-    for (int s = 0; s < 10; ++s) f[14 + s] = (float)(sc[s] * S[s]);
-    f[24] = (float)(-4.5 * (S[0] + S[5] + S[7]));
-    f[25] = (float)(-4.5 * (S[3] + S[1] + S[8]));
-    f[26] = (float)(-4.5 * (S[4] + S[6] + S[2]));
+    const double cx = -4.5 * (S[0] + S[5] + S[7]);
+    const double cy = -4.5 * (S[3] + S[1] + S[8]);
+    const double cz = -4.5 * (S[4] + S[6] + S[2]);
+    /* monomials: xxx yyy zzz xxy xxz yyx yyz zzx zzy xyz */
+    const double fold[10] = {cx, cy, cz, cy, cz, cx, cz, cx, cy, 0.0};
+    for (int s = 0; s < 10; ++s) f[14 + s] = (float)(sc[s] * S[s] + fold[s]);
+    f[24] = (float)cx;
+    f[25] = (float)cy;
+    f[26] = (float)cz;
     memcpy(w, f, sizeof(f));
 }
 
@@ -598,8 +607,9 @@ static inline double tri_f64_words(const float* t, float qx, float qy, float qz)
     return tri_f64(a, b, c);
 }
 
-/* omega + inv3 (o0 + inv2 (q1 + inv2 q2)) with q1 and q2 in Horner form: the GPU kernel's
- * operation order (one rounding per FMA), so the same tree gives the same sum up to rsqrt. */
+/* omega + inv3 (o0 + inv2 (q1 + inv2 q2)), o0 = d.N, q1 = d^T A d and q2 = B(d, d, d) homogeneous
+ * (tr and C folded into A and B by slot_from_moments), in Horner form: the GPU kernel's operation
+ * order (one rounding per FMA), so the same tree gives the same sum up to rsqrt. */
 static inline float far_field_acc(const float* w, float dx, float dy, float dz, float d2, float om)
 {
     const float inv = 1.0f / sqrtf(d2);
@@ -607,17 +617,17 @@ static inline float far_field_acc(const float* w, float dx, float dy, float dz, 
     const float inv3 = inv2 * inv;
     const float* A = w + FWN_W_A;
     const float* Bq = w + FWN_W_B;
-    const float* Cq = w + FWN_W_C;
-    const float o0 = fmaf(dx, w[FWN_W_NX], fmaf(dy, w[FWN_W_NY], fmaf(dz, w[FWN_W_NZ], w[FWN_W_TR])));
-    const float a1 = fmaf(A[0], dx, fmaf(A[3], dy, fmaf(A[5], dz, Cq[0])));
-    const float b1 = fmaf(A[1], dy, fmaf(A[4], dz, Cq[1]));
-    const float c1 = fmaf(A[2], dz, Cq[2]);
+    const float o0 = fmaf(dx, w[FWN_W_NX], fmaf(dy, w[FWN_W_NY], dz * w[FWN_W_NZ]));
+    const float a1 = fmaf(A[0], dx, fmaf(A[3], dy, A[5] * dz));
+    const float b1 = fmaf(A[1], dy, A[4] * dz);
+    const float c1 = A[2] * dz;
     const float q1 = fmaf(dx, a1, fmaf(dy, b1, dz * c1));
-    const float ax = fmaf(Bq[0], dx, fmaf(Bq[3], dy, Bq[4] * dz));
-    const float bx = fmaf(Bq[5], dy, Bq[9] * dz);
-    const float X = fmaf(dx, ax, fmaf(dy, bx, dz * (Bq[7] * dz)));
-    const float Y = fmaf(dy, fmaf(Bq[1], dy, Bq[6] * dz), dz * (Bq[8] * dz));
-    const float Z = dz * (dz * Bq[2]);
+    const float zz = dz * dz;
+    const float P = fmaf(Bq[0], dx, fmaf(Bq[3], dy, Bq[4] * dz));
+    const float Qv = fmaf(Bq[5], dy, Bq[9] * dz);
+    const float X = fmaf(dx, P, fmaf(dy, Qv, Bq[7] * zz));
+    const float Y = fmaf(dy, fmaf(Bq[1], dy, Bq[6] * dz), Bq[8] * zz);
+    const float Z = Bq[2] * zz;
     const float q2 = fmaf(dx, X, fmaf(dy, Y, dz * Z));
     const float t = fmaf(inv2, fmaf(inv2, q2, q1), o0);
     return fmaf(t, inv3, om);

@@ -132,6 +132,9 @@ def _pack_slot(rec):
     f[24] = -4.5 * (S[0] + S[5] + S[7])
     f[25] = -4.5 * (S[3] + S[1] + S[8])
     f[26] = -4.5 * (S[4] + S[6] + S[2])
+    # tr joins the quadratic's diagonal, C the cubic monomials xxx yyy zzz xxy xxz yyx yyz zzx zzy
+    f[8:11] += f[7]
+    f[14:23] += f[[24, 25, 26, 25, 26, 24, 26, 24, 25]]
     w = f.view(np.uint32).copy()
     w[27:32] = np.array([-1, 0, 1, 0, 0], np.int32).view(np.uint32)
     return w
