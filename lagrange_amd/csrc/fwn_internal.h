@@ -134,8 +134,10 @@ struct Engine {
 void radix_sort_pairs(uint64_t*& keys, uint32_t*& vals, uint64_t*& keys_alt, uint32_t*& vals_alt, int64_t n,
                       int nbits, DBuf& hist, DBuf& rowsum, hipStream_t s);
 // Same for 32-bit keys, sorting key bits [begin_bit, end_bit) only (low bits keep input order).
+// vals_are_index: the input values are not read; value i is taken to be i (the sorted values are
+// then the permutation), so the key kernel need not write them.
 void radix_sort_pairs32(uint32_t*& keys, uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt, int64_t n,
-                        int begin_bit, int end_bit, DBuf& hist, DBuf& rowsum, hipStream_t s);
+                        int begin_bit, int end_bit, bool vals_are_index, DBuf& hist, DBuf& rowsum, hipStream_t s);
 // exclusive scan of int32 (in may alias out); returns nothing, total in out[n] if out has n+1 slots
 void exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, DBuf& tmp, hipStream_t s);
 

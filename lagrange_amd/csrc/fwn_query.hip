@@ -50,8 +50,7 @@ __device__ __forceinline__ int mesh_of64(const int64_t* off, int M, int64_t t)
 // which points share its wave.
 __global__ __launch_bounds__(QT) void k_query_keys(const float* __restrict__ Q, int64_t nQ,
                                                    const int64_t* __restrict__ q_off, int M, int mesh_bits,
-                                                   const float* __restrict__ minfo, uint32_t* __restrict__ keys,
-                                                   uint32_t* __restrict__ vals)
+                                                   const float* __restrict__ minfo, uint32_t* __restrict__ keys)
 {
     int64_t i = (int64_t)blockIdx.x * QT + threadIdx.x;
     if (i >= nQ) return;
@@ -61,8 +60,7 @@ __global__ __launch_bounds__(QT) void k_query_keys(const float* __restrict__ Q, 
     const int mbits = 32 - mesh_bits; // bits left for the Morton code
     uint32_t key = (mbits >= 30) ? mc : (mc >> (30 - mbits));
     if (mesh_bits > 0) key |= (uint32_t)m << mbits;
-    keys[i] = key;
-    vals[i] = (uint32_t)i;
+    keys[i] = key; // the sort's first pass takes the values to be the positions i
 }
 
 // exact triangle solid angle, FP32, normalised-vector van Oosterom-Strackee form
@@ -593,9 +591,9 @@ void run_queries_pipelined(Engine& e, const float* Q, int64_t nQ, float* omega, 
             uint32_t* kp2 = e.q_keys[1].as<uint32_t>();
             uint32_t* vp2 = e.q_vals[1].as<uint32_t>();
             hipLaunchKernelGGL(k_query_keys, dim3(grid_blocks(ck, QT)), dim3(QT), 0, s, dQ, ck, d_qoff, 1, 0,
-                               e.mesh_info.as<float>(), kp, vp);
+                               e.mesh_info.as<float>(), kp);
             FWN_LAUNCH_CHECK("k_query_keys");
-            radix_sort_pairs32(kp, vp, kp2, vp2, ck, 8, 32, e.q_hist, e.q_rowsum, s);
+            radix_sort_pairs32(kp, vp, kp2, vp2, ck, 8, 32, true, e.q_hist, e.q_rowsum, s);
             const int64_t nb = (ck + 63) / 64;
             int64_t nblk = (nb + waves - 1) / waves;
             if (persist) nblk = std::min<int64_t>(nblk, (int64_t)e.n_cu * resident_blocks(true, waves, banks));
@@ -767,9 +765,9 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
         while ((1 << mesh_bits) < M) ++mesh_bits;
         if (mesh_bits > 24) fail(FWN_LIMIT_EXCEEDED, "more than 2^24 meshes in one engine");
         hipLaunchKernelGGL(k_query_keys, dim3(grid_blocks(nQ, QT)), dim3(QT), 0, s, dQ, nQ, d_qoff, M, mesh_bits,
-                           e.mesh_info.as<float>(), kp, vp);
+                           e.mesh_info.as<float>(), kp);
         FWN_LAUNCH_CHECK("k_query_keys");
-        radix_sort_pairs32(kp, vp, kp2, vp2, nQ, 8, 32, e.q_hist, e.q_rowsum, s);
+        radix_sort_pairs32(kp, vp, kp2, vp2, nQ, 8, 32, true, e.q_hist, e.q_rowsum, s);
     }
 
     const int waves = e.opt.waves_per_block > 0 ? e.opt.waves_per_block : 1;
@@ -855,9 +853,9 @@ const uint32_t* sort_points_single(Engine& e, const float* dQ, int64_t nQ)
     uint32_t* kp2 = e.q_keys[1].as<uint32_t>();
     uint32_t* vp2 = e.q_vals[1].as<uint32_t>();
     hipLaunchKernelGGL(k_query_keys, dim3(grid_blocks(nQ, QT)), dim3(QT), 0, s, dQ, nQ, (const int64_t*)nullptr, 1, 0,
-                       e.mesh_info.as<float>(), kp, vp);
+                       e.mesh_info.as<float>(), kp);
     FWN_LAUNCH_CHECK("k_query_keys");
-    radix_sort_pairs32(kp, vp, kp2, vp2, nQ, 8, 32, e.q_hist, e.q_rowsum, s);
+    radix_sort_pairs32(kp, vp, kp2, vp2, nQ, 8, 32, true, e.q_hist, e.q_rowsum, s);
     return vp;
 }
 

@@ -3,13 +3,15 @@
 // query points (wave batches of nearby points).
 //
 // One pass = 8 key bits:
-//   hist:    each 256-thread block counts the digits of its 4096-key tile in LDS
+//   hist:    each 256-thread block counts the digits of its 4096-key tile in LDS (the tile's
+//            loads are all issued before the first count)
 //   rows:    one block per digit scans that digit's per-block counts (digit-major layout)
 //   digits:  one block scans the 256 digit totals
-//   scatter: each block re-reads its tile in 16 rounds of 256 keys; inside a wave the rank of a
-//            key among equal digits comes from 8 ballots, across the block's 4 waves from LDS
-//            counts, and the running per-digit offset keeps the order stable; the tile is
-//            placed in LDS in digit order first, then written out in coalesced runs.
+//   scatter: each block loads its tile into registers; each wave ranks its quarter of the tile
+//            in 16 rounds of 64 keys (8 ballots give a key's rank among the wave's equal
+//            digits, wave-private LDS run counters its place in the wave's run), one block step
+//            turns the waves' runs into stable tile offsets, the tile is placed in LDS in digit
+//            order and written out in coalesced runs.
 #include "fwn_internal.h"
 
 namespace fwn {
@@ -21,17 +23,54 @@ constexpr int RS_ROUNDS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;
 constexpr int RS_WAVES = RS_THREADS / kWave;
 
+// lanes of the wave holding the same 8-bit digit d as this lane (among the lanes in `live`, a
+// prefix of the wave).  Morton keys of nearby points often share a digit across the whole wave:
+// one compare settles that case; otherwise 8 bit ballots separate the digits.  Called with every
+// lane of the wave active.
+__device__ __forceinline__ uint64_t digit_peers(uint32_t d, uint64_t live)
+{
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    if ((__ballot(d != d0) & live) == 0ull) return live;
+    uint64_t peers = live;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t m = __ballot(bit);
+        peers &= bit ? m : ~m;
+    }
+    return peers;
+}
+
 template <typename KeyT>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const KeyT* __restrict__ keys, int64_t n, int shift,
                                                         int32_t* __restrict__ hist, int nblocks)
 {
     __shared__ int32_t cnt[256];
     cnt[threadIdx.x] = 0;
-    __syncthreads();
-    int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    const int lane = threadIdx.x & (kWave - 1);
+    // all of the thread's loads are issued before the first count (one memory latency per tile)
+    KeyT kr[RS_ROUNDS];
+#pragma unroll
     for (int r = 0; r < RS_ROUNDS; ++r) {
-        int64_t i = base + r * RS_THREADS + threadIdx.x;
-        if (i < n) atomicAdd(&cnt[(uint32_t)(keys[i] >> shift) & 255u], 1);
+        const int64_t i = base + r * RS_THREADS + threadIdx.x;
+        kr[r] = (i < n) ? keys[i] : (KeyT)0;
+    }
+    __syncthreads();
+    // a wave whose keys share one digit adds its count with one atomic (LDS atomics on one
+    // address serialise); other waves count key by key
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const int64_t i = base + r * RS_THREADS + threadIdx.x;
+        const bool valid = i < n;
+        const uint32_t d = (uint32_t)(kr[r] >> shift) & 255u;
+        const uint64_t live = __ballot(valid);
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+        if ((__ballot(d != d0) & live) == 0ull) {
+            if (lane == 0 && live != 0ull) atomicAdd(&cnt[d0], (int32_t)__popcll(live));
+        } else if (valid) {
+            atomicAdd(&cnt[d], 1);
+        }
     }
     __syncthreads();
     hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
@@ -77,9 +116,14 @@ __global__ __launch_bounds__(256) void k_rs_digits(int32_t* __restrict__ rowsum)
     rowsum[threadIdx.x] = part[threadIdx.x] - v;
 }
 
-// The tile is first ordered by digit in LDS (stable), then written out: consecutive threads
-// write consecutive positions of a digit's run, so the global stores coalesce.  The output is
-// the same as writing each key straight to its position.
+// Wave w of the block owns keys [w * 1024, (w + 1) * 1024) of the tile.  Each wave ranks its
+// keys on its own (no block barrier per round): per round of 64 keys the 8 digit ballots give a
+// key's rank among the wave's equal digits, and wave-private run counters in LDS give its place
+// in the wave's run of that digit.  One block step then turns the four waves' run lengths into
+// tile offsets (digit-major, wave order inside a digit: stable), the keys are placed in LDS in
+// digit order, and the tile is written out in coalesced runs: consecutive threads write
+// consecutive positions of a digit's run.  The output is the same as writing each key straight
+// to its position.
 template <typename KeyT>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const KeyT* __restrict__ kin,
                                                            const uint32_t* __restrict__ vin,
@@ -87,10 +131,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const KeyT* __restric
                                                            int64_t n, int shift, const int32_t* __restrict__ hist,
                                                            const int32_t* __restrict__ digit_base, int nblocks)
 {
-    __shared__ int32_t gbase[256];   // global position of this tile's first key of each digit
-    __shared__ int32_t toff[256];    // tile-local start of each digit's run
-    __shared__ int32_t lrun[256];    // keys of each digit already placed in earlier rounds
-    __shared__ int32_t wc[RS_WAVES][256];
+    constexpr int kWaveKeys = RS_TILE / RS_WAVES; // 1024
+    __shared__ int32_t gbase[256];          // global position of this tile's first key of each digit
+    __shared__ int32_t toff[256];           // tile-local start of each digit's run
+    __shared__ int32_t wrun[RS_WAVES][256]; // per wave: keys of each digit (run length, then offset)
     __shared__ KeyT lk[RS_TILE];
     __shared__ uint32_t lv[RS_TILE];
     const int tid = threadIdx.x;
@@ -98,6 +142,17 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const KeyT* __restric
     const int wave = tid / kWave;
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
     const int tile_n = (int)((n - base) < RS_TILE ? (n - base) : RS_TILE);
+    // the tile's keys and values are loaded into registers first (the loads overlap the offset
+    // scan below), so the block waits for memory once rather than once per round
+    KeyT kr[RS_ROUNDS];
+    uint32_t vr[RS_ROUNDS];
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const int li = wave * kWaveKeys + r * kWave + lane;
+        kr[r] = (li < tile_n) ? kin[base + li] : (KeyT)0;
+        // vin == nullptr: the values are the input positions (a first pass over unsorted keys)
+        vr[r] = (li < tile_n) ? (vin ? vin[base + li] : (uint32_t)(base + li)) : 0u;
+    }
     // this tile's count of digit tid: the next block's exclusive offset (or the digit total) minus ours
     const int32_t mine = hist[(int64_t)tid * nblocks + blockIdx.x];
     int32_t next;
@@ -105,6 +160,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const KeyT* __restric
     else next = ((tid < 255) ? digit_base[tid + 1] : (int32_t)n) - digit_base[tid];
     const int32_t cnt = next - mine;
     gbase[tid] = mine + digit_base[tid];
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; ++w) wrun[w][tid] = 0;
     // exclusive scan of the tile's digit counts (Hillis-Steele in LDS)
     toff[tid] = cnt;
     __syncthreads();
@@ -115,38 +172,46 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const KeyT* __restric
         __syncthreads();
     }
     toff[tid] -= cnt;
-    lrun[tid] = 0;
+    // each wave: the place of every key in the wave's run of its digit
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // volatile: each round reloads the counters other lanes of the wave wrote in earlier rounds
+    volatile int32_t* run_of = wrun[wave];
+    int32_t lp[RS_ROUNDS];
+#pragma unroll
     for (int r = 0; r < RS_ROUNDS; ++r) {
-        if (r * RS_THREADS >= tile_n) break; // uniform across the block
-        for (int k = tid; k < RS_WAVES * 256; k += RS_THREADS) (&wc[0][0])[k] = 0;
-        const int li = r * RS_THREADS + tid;
-        const bool valid = li < tile_n;
-        const KeyT key = valid ? kin[base + li] : (KeyT)0;
-        const uint32_t val = valid ? vin[base + li] : 0u;
-        const uint32_t d = (uint32_t)(key >> shift) & 255u;
-        uint64_t peers = __ballot(valid);
-        for (int b = 0; b < 8; ++b) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t m = __ballot(bit);
-            peers &= bit ? m : ~m;
-        }
+        const bool valid = wave * kWaveKeys + r * kWave + lane < tile_n;
+        const uint32_t d = (uint32_t)(kr[r] >> shift) & 255u;
+        const uint64_t peers = digit_peers(d, __ballot(valid));
         const int rank = __popcll(peers & lt);
-        __syncthreads();
-        if (valid && rank == 0) wc[wave][d] = __popcll(peers);
-        __syncthreads();
-        if (valid) {
-            int32_t pos = toff[d] + lrun[d] + rank;
-            for (int w = 0; w < wave; ++w) pos += wc[w][d];
-            lk[pos] = key;
-            lv[pos] = val;
-        }
-        __syncthreads();
-        int32_t add = 0;
-        for (int w = 0; w < RS_WAVES; ++w) add += wc[w][tid];
-        lrun[tid] += add;
-        __syncthreads();
+        const int32_t run = valid ? run_of[d] : 0;
+        lp[r] = run + rank;
+        // the wave's LDS operations complete in order, and the write needs the value read:
+        // every peer has read the counter before the lowest peer writes it
+        if (valid && rank == 0) run_of[d] = run + (int32_t)__popcll(peers);
     }
+    __syncthreads();
+    // run lengths -> tile offsets of each wave's run of digit tid
+    {
+        int32_t at = toff[tid];
+#pragma unroll
+        for (int w = 0; w < RS_WAVES; ++w) {
+            const int32_t len = wrun[w][tid];
+            wrun[w][tid] = at;
+            at += len;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const bool valid = wave * kWaveKeys + r * kWave + lane < tile_n;
+        if (valid) {
+            const uint32_t d = (uint32_t)(kr[r] >> shift) & 255u;
+            const int32_t pos = wrun[wave][d] + lp[r];
+            lk[pos] = kr[r];
+            lv[pos] = vr[r];
+        }
+    }
+    __syncthreads();
     for (int t = tid; t < tile_n; t += RS_THREADS) {
         const KeyT key = lk[t];
         const uint32_t d = (uint32_t)(key >> shift) & 255u;
@@ -221,11 +286,23 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_add(int32_t* __restrict__ o
     if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = tile_sums[gridDim.x];
 }
 
+__global__ __launch_bounds__(256) void k_iota(uint32_t* __restrict__ v, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) v[i] = (uint32_t)i;
+}
+
 template <typename KeyT>
 void radix_sort_impl(KeyT*& keys, uint32_t*& vals, KeyT*& keys_alt, uint32_t*& vals_alt, int64_t n, int begin_bit,
-                     int end_bit, DBuf& hist, DBuf& rowsum, hipStream_t s)
+                     int end_bit, bool vals_are_index, DBuf& hist, DBuf& rowsum, hipStream_t s)
 {
-    if (n <= 1) return;
+    if (n <= 1 || begin_bit >= end_bit) {
+        if (n >= 1 && vals_are_index) {
+            hipLaunchKernelGGL(k_iota, dim3(grid_blocks(n, 256)), dim3(256), 0, s, vals, n);
+            FWN_LAUNCH_CHECK("k_iota");
+        }
+        return;
+    }
     if (n > (int64_t)INT32_MAX) fail(FWN_LIMIT_EXCEEDED, "radix sort: more than 2^31-1 elements");
     int nblocks = (int)((n + RS_TILE - 1) / RS_TILE);
     hist.reserve(sizeof(int32_t) * 256 * (size_t)nblocks);
@@ -239,7 +316,8 @@ void radix_sort_impl(KeyT*& keys, uint32_t*& vals, KeyT*& keys_alt, uint32_t*& v
         FWN_LAUNCH_CHECK("k_rs_rows");
         hipLaunchKernelGGL(k_rs_digits, dim3(1), dim3(256), 0, s, rowsum.as<int32_t>());
         FWN_LAUNCH_CHECK("k_rs_digits");
-        hipLaunchKernelGGL(k_rs_scatter<KeyT>, dim3(nblocks), dim3(RS_THREADS), 0, s, keys, vals, keys_alt,
+        const uint32_t* vin = (vals_are_index && shift == begin_bit) ? nullptr : vals;
+        hipLaunchKernelGGL(k_rs_scatter<KeyT>, dim3(nblocks), dim3(RS_THREADS), 0, s, keys, vin, keys_alt,
                            vals_alt, n, shift, hist.as<int32_t>(), rowsum.as<int32_t>(), nblocks);
         FWN_LAUNCH_CHECK("k_rs_scatter");
         std::swap(keys, keys_alt);
@@ -252,13 +330,13 @@ void radix_sort_impl(KeyT*& keys, uint32_t*& vals, KeyT*& keys_alt, uint32_t*& v
 void radix_sort_pairs(uint64_t*& keys, uint32_t*& vals, uint64_t*& keys_alt, uint32_t*& vals_alt, int64_t n,
                       int nbits, DBuf& hist, DBuf& rowsum, hipStream_t s)
 {
-    radix_sort_impl<uint64_t>(keys, vals, keys_alt, vals_alt, n, 0, nbits, hist, rowsum, s);
+    radix_sort_impl<uint64_t>(keys, vals, keys_alt, vals_alt, n, 0, nbits, false, hist, rowsum, s);
 }
 
 void radix_sort_pairs32(uint32_t*& keys, uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt, int64_t n,
-                        int begin_bit, int end_bit, DBuf& hist, DBuf& rowsum, hipStream_t s)
+                        int begin_bit, int end_bit, bool vals_are_index, DBuf& hist, DBuf& rowsum, hipStream_t s)
 {
-    radix_sort_impl<uint32_t>(keys, vals, keys_alt, vals_alt, n, begin_bit, end_bit, hist, rowsum, s);
+    radix_sort_impl<uint32_t>(keys, vals, keys_alt, vals_alt, n, begin_bit, end_bit, vals_are_index, hist, rowsum, s);
 }
 
 // out must hold n+1 entries; out[n] receives the total.

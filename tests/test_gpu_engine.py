@@ -34,8 +34,29 @@ def slot_view(slots):
     return s.view(np.float32), s.view(np.int32)
 
 
-def test_sorted_triangle_order_is_morton_order(oracle, ico5):
-    V, F = ico5
+def _dup_soup(n_tiles_plus):
+    """Triangles whose Morton keys repeat heavily (runs of equal keys span sort tiles of 4,096
+    keys) and a count that leaves a partial last tile."""
+    rng = np.random.default_rng(17)
+    base = rng.uniform(-1, 1, (40, 3, 3)).astype(np.float32)
+    reps = rng.integers(1, 700, 40)
+    tri = np.repeat(base, reps, axis=0)
+    tri = tri[:n_tiles_plus] if tri.shape[0] >= n_tiles_plus else np.concatenate([tri, tri])[:n_tiles_plus]
+    V = tri.reshape(-1, 3)
+    F = np.arange(V.shape[0], dtype=np.int32).reshape(-1, 3)
+    return V, F
+
+
+@pytest.mark.parametrize("case", ["ico5", "open200k", "dup_soup"])
+def test_sorted_triangle_order_is_morton_order(oracle, ico5, case):
+    """The stable radix sort behind the LBVH order: ties keep facet order, many tiles, a partial
+    last tile, tiles whose keys all share a digit."""
+    if case == "ico5":
+        V, F = ico5
+    elif case == "open200k":
+        V, F = W.displaced_sphere(100, 4, amplitude=0.15, delete_fraction=0.05).f32()
+    else:
+        V, F = _dup_soup(3 * 4096 + 17)
     e = FastWindingNumber((V, F), tree_builder=L.FWN_TREE_LBVH)
     _, tris, _, _ = e.export_tree()
     got = tris.reshape(-1, 12)[:, 9].astype(np.int64)
