@@ -5,7 +5,8 @@
 // One pass = 8 key bits:
 //   hist:    each 256-thread block counts the digits of its 4096-key tile in LDS (the tile's
 //            loads are all issued before the first count)
-//   rows:    one block per digit scans that digit's per-block counts (digit-major layout)
+//   rows:    one block per digit scans that digit's per-block counts (digit-major layout) in
+//            chunks of 4,096 staged through LDS
 //   digits:  one block scans the 256 digit totals
 //   scatter: each block loads its tile into registers; each wave ranks its quarter of the tile
 //            in 16 rounds of 64 keys (8 ballots give a key's rank among the wave's equal
@@ -76,29 +77,57 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const KeyT* __restrict__
     hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// exclusive scan of one digit's row of per-block counts; row total -> rowsum[d]
+// exclusive scan of one digit's row of per-block counts; row total -> rowsum[d].  The row is
+// taken in chunks of 4,096 counts: loaded into LDS (coalesced), each thread sums 16 consecutive
+// counts, one block scan of the 256 sums, then each thread writes its 16 offsets.
+constexpr int RR_PER = 16;
+constexpr int RR_CHUNK = 256 * RR_PER;
+__device__ __forceinline__ int rr_pad(int i) { return i + i / RR_PER; } // odd stride: no bank conflicts
+
 __global__ __launch_bounds__(256) void k_rs_rows(int32_t* __restrict__ hist, int nblocks,
                                                  int32_t* __restrict__ rowsum)
 {
+    __shared__ int32_t buf[RR_CHUNK + RR_CHUNK / RR_PER];
     __shared__ int32_t part[256];
+    const int tid = threadIdx.x;
     int32_t* row = hist + (int64_t)blockIdx.x * nblocks;
     int32_t carry = 0;
-    for (int base = 0; base < nblocks; base += 256) {
-        int i = base + threadIdx.x;
-        int32_t v = (i < nblocks) ? row[i] : 0;
-        part[threadIdx.x] = v;
+    for (int base = 0; base < nblocks; base += RR_CHUNK) {
+#pragma unroll
+        for (int k = 0; k < RR_PER; ++k) {
+            const int i = k * 256 + tid;
+            buf[rr_pad(i)] = (base + i < nblocks) ? row[base + i] : 0;
+        }
+        __syncthreads();
+        int32_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < RR_PER; ++k) sum += buf[rr_pad(tid * RR_PER + k)];
+        part[tid] = sum;
         __syncthreads();
         for (int off = 1; off < 256; off <<= 1) {
-            int32_t t = (threadIdx.x >= (unsigned)off) ? part[threadIdx.x - off] : 0;
+            const int32_t t = (tid >= off) ? part[tid - off] : 0;
             __syncthreads();
-            part[threadIdx.x] += t;
+            part[tid] += t;
             __syncthreads();
         }
-        if (i < nblocks) row[i] = carry + part[threadIdx.x] - v;
+        int32_t run = carry + part[tid] - sum;
+#pragma unroll
+        for (int k = 0; k < RR_PER; ++k) {
+            const int j = rr_pad(tid * RR_PER + k);
+            const int32_t v = buf[j];
+            buf[j] = run;
+            run += v;
+        }
         carry += part[255];
         __syncthreads();
+#pragma unroll
+        for (int k = 0; k < RR_PER; ++k) {
+            const int i = k * 256 + tid;
+            if (base + i < nblocks) row[base + i] = buf[rr_pad(i)];
+        }
+        __syncthreads();
     }
-    if (threadIdx.x == 0) rowsum[blockIdx.x] = carry;
+    if (tid == 0) rowsum[blockIdx.x] = carry;
 }
 
 __global__ __launch_bounds__(256) void k_rs_digits(int32_t* __restrict__ rowsum)

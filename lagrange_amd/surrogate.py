@@ -109,11 +109,11 @@ class Surrogate:
             import torch
             q = points.contiguous().to(torch.float32)
             w = torch.empty(q.shape[0], dtype=torch.float32, device=q.device)
-            ins = torch.empty(q.shape[0], dtype=torch.uint8, device=q.device)
+            ins = torch.empty(q.shape[0], dtype=torch.bool, device=q.device)
             torch.cuda.synchronize(q.device)
             L.check(lib.fwn_surrogate_query(self._h, _ptr(q), q.shape[0], _ptr(w), _ptr(ins), L.FWN_DEVICE_POINTERS,
                                             C.byref(nf)))
-            return w, ins.bool(), nf.value
+            return w, ins, nf.value
         q = np.ascontiguousarray(np.asarray(points, np.float32).reshape(-1, 3))
         w = np.empty(q.shape[0], np.float32)
         ins = np.empty(q.shape[0], np.uint8)

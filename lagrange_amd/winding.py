@@ -203,11 +203,12 @@ class FastWindingNumber(_EngineBase):
             q = points.contiguous().to(torch.float32)
             n = q.shape[0]
             om = torch.empty(n, dtype=torch.float32, device=q.device)
-            ins = torch.empty(n, dtype=torch.uint8, device=q.device) if with_inside else None
+            # a torch.bool element is one byte holding 0 or 1: the engine's flag bytes, written in place
+            ins = torch.empty(n, dtype=torch.bool, device=q.device) if with_inside else None
             torch.cuda.synchronize(q.device)
             L.check(L.load().fwn_solid_angle(self._h(), _ptr(q), n, _ptr(om), _ptr(ins) if ins is not None else None,
                                              L.FWN_DEVICE_POINTERS))
-            return (om, ins.bool()) if with_inside else om
+            return (om, ins) if with_inside else om
         q = np.ascontiguousarray(np.asarray(points, dtype=np.float32).reshape(-1, 3))
         n = q.shape[0]
         om = np.empty(n, np.float32)
@@ -234,13 +235,13 @@ class FastWindingNumber(_EngineBase):
         shape = (int(dims_a[2]), int(dims_a[1]), int(dims_a[0]))
         if device is not None:
             import torch
-            ins = torch.empty(n, dtype=torch.uint8, device=device)
+            ins = torch.empty(n, dtype=torch.bool, device=device)
             om = torch.empty(n, dtype=torch.float32, device=device) if with_omega else None
             torch.cuda.synchronize(device)
             L.check(L.load().fwn_sign_voxel_box(self._h(), lo_a.ctypes.data, dims_a.ctypes.data, float(voxel_size),
                                                 _ptr(ins), _ptr(om) if om is not None else None,
                                                 L.FWN_DEVICE_POINTERS))
-            ins = ins.bool().reshape(shape)
+            ins = ins.reshape(shape)
             return (ins, om.reshape(shape)) if with_omega else ins
         ins = np.empty(n, np.uint8)
         om = np.empty(n, np.float32) if with_omega else None
@@ -256,12 +257,12 @@ class FastWindingNumber(_EngineBase):
             import torch
             v = ijk.contiguous().to(torch.int32)
             n = v.shape[0]
-            ins = torch.empty(n, dtype=torch.uint8, device=v.device)
+            ins = torch.empty(n, dtype=torch.bool, device=v.device)
             om = torch.empty(n, dtype=torch.float32, device=v.device) if with_omega else None
             torch.cuda.synchronize(v.device)
             L.check(L.load().fwn_sign_voxels(self._h(), _ptr(v), n, float(voxel_size), _ptr(ins),
                                              _ptr(om) if om is not None else None, L.FWN_DEVICE_POINTERS))
-            return (ins.bool(), om) if with_omega else ins.bool()
+            return (ins, om) if with_omega else ins
         v = np.ascontiguousarray(np.asarray(ijk, dtype=np.int32).reshape(-1, 3))
         n = v.shape[0]
         ins = np.empty(n, np.uint8)
@@ -418,11 +419,12 @@ class MultiFastWindingNumber(_EngineBase):
             q = points.contiguous().to(torch.float32)
             n = q.shape[0]
             om = torch.empty(n, dtype=torch.float32, device=q.device)
-            ins = torch.empty(n, dtype=torch.uint8, device=q.device) if with_inside else None
+            # a torch.bool element is one byte holding 0 or 1: the engine's flag bytes, written in place
+            ins = torch.empty(n, dtype=torch.bool, device=q.device) if with_inside else None
             torch.cuda.synchronize(q.device)
             L.check(L.load().fwn_solid_angle_multi(self._h(), _ptr(q), q_off.ctypes.data, _ptr(om),
                                                    _ptr(ins) if ins is not None else None, L.FWN_DEVICE_POINTERS))
-            return (om, ins.bool()) if with_inside else om
+            return (om, ins) if with_inside else om
         q = np.ascontiguousarray(np.asarray(points, dtype=np.float32).reshape(-1, 3))
         om = np.empty(q.shape[0], np.float32)
         ins = np.empty(q.shape[0], np.uint8) if with_inside else None
