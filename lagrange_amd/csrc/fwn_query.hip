@@ -109,8 +109,9 @@ __device__ __forceinline__ double tri_f64(const float* __restrict__ t, float qx,
     return 2.0 * atan2(num, den);
 }
 
-// One slot's 32 words held in SGPRs (two 16-dword scalar loads).  The traversal issues the
-// next slot's loads before the current slot's arithmetic so the load latency overlaps it.
+// One slot's 32 words held in SGPRs (two 16-dword scalar loads issued together at the top of
+// the slot).  The other waves of the SIMD (8 resident) hide the load latency; a prefetch of the
+// next slot into a second register set costs 16 scalar moves per slot and 24 more SGPRs.
 typedef uint32_t v16u __attribute__((ext_vector_type(16)));
 struct SlotRegs {
     v16u a, b;
@@ -293,14 +294,7 @@ __device__ __forceinline__ void traverse_batch(
                     }
                 }
             };
-            // the next slot's scalar loads are issued before the current slot's arithmetic
-            SlotRegs cur = load_slot(nb);
-            for (int s = 0; s < ns; ++s) {
-                SlotRegs nxt = cur;
-                if (s + 1 < ns) nxt = load_slot(nb + (s + 1) * kSlotWords);
-                slot(cur);
-                cur = nxt;
-            }
+            for (int s = 0; s < ns; ++s) slot(load_slot(nb + s * kSlotWords));
         }
         if (overflow && lane == 0) atomicOr(flag, 1);
     }
