@@ -115,6 +115,7 @@ struct Engine {
     void* h_stage = nullptr;
     size_t h_stage_bytes = 0;
     DBuf q_meta; // device copy of the staged metadata (and small host batches' points)
+    std::vector<char> meta_dev; // the metadata bytes q_meta holds (compared before an upload)
     // pipelined host calls: pinned double buffers (points in, results out), a stream per copy
     // direction, per-chunk flags
     void* h_pipe = nullptr;

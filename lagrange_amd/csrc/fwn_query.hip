@@ -697,8 +697,16 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
     std::memcpy(hs + sizeof(int64_t) * (M + 1), boff.data(), sizeof(int64_t) * (M + 1));
     std::memcpy(hs + sizeof(int64_t) * 2 * (M + 1), order.data(), sizeof(int32_t) * M);
     if (stage_io) std::memcpy(hs + meta_b, Q, sizeof(float) * 3 * (size_t)nQ);
+    const void* meta_before = e.q_meta.p;
     e.q_meta.reserve(in_b);
-    FWN_HIP(hipMemcpyAsync(e.q_meta.p, hs, in_b, hipMemcpyHostToDevice, s));
+    // repeated calls of one shape (same meshes, point counts and batch size) find the metadata
+    // already on the device and skip the upload
+    const bool meta_on_device = !stage_io && e.q_meta.p == meta_before && e.meta_dev.size() == meta_b &&
+                                std::memcmp(e.meta_dev.data(), hs, meta_b) == 0;
+    if (!meta_on_device) {
+        FWN_HIP(hipMemcpyAsync(e.q_meta.p, hs, in_b, hipMemcpyHostToDevice, s));
+        e.meta_dev.assign(hs, hs + meta_b);
+    }
     const int64_t* d_qoff = e.q_meta.as<int64_t>();
     const int64_t* d_boff = d_qoff + (M + 1);
     const int32_t* d_order = reinterpret_cast<const int32_t*>(d_qoff + 2 * (M + 1));
