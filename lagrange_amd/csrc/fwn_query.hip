@@ -813,9 +813,10 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
             if (inside) FWN_HIP(hipMemcpyAsync(hs + 16 + sizeof(float) * nQ, dIn, nQ, hipMemcpyDeviceToHost, s));
             FWN_HIP(hipStreamSynchronize(s));
             std::memcpy(h_fl, hs, sizeof(h_fl));
-        } else {
-            FWN_HIP(hipMemcpyAsync(h_fl, dflag, sizeof(h_fl), hipMemcpyDeviceToHost, s));
+        } else { // through the pinned stage too (its metadata bytes were uploaded before, in order)
+            FWN_HIP(hipMemcpyAsync(hs, dflag, sizeof(h_fl), hipMemcpyDeviceToHost, s));
             FWN_HIP(hipStreamSynchronize(s));
+            std::memcpy(h_fl, hs, sizeof(h_fl));
         }
         if (h_fl[0]) fail(FWN_LIMIT_EXCEEDED, "traversal stack overflow");
         const int64_t need = (int64_t)(uint32_t)h_fl[1] * kItemChunk; // claimed chunks x slots

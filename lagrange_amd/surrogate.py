@@ -12,7 +12,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib as L
-from .winding import FastWindingNumber, _is_torch, _ptr
+from .winding import FastWindingNumber, _is_torch, _ptr, _torch_ready
 
 
 class SurrogateOptions(C.Structure):
@@ -110,7 +110,7 @@ class Surrogate:
             q = points.contiguous().to(torch.float32)
             w = torch.empty(q.shape[0], dtype=torch.float32, device=q.device)
             ins = torch.empty(q.shape[0], dtype=torch.bool, device=q.device)
-            torch.cuda.synchronize(q.device)
+            _torch_ready(q.device)
             L.check(lib.fwn_surrogate_query(self._h, _ptr(q), q.shape[0], _ptr(w), _ptr(ins), L.FWN_DEVICE_POINTERS,
                                             C.byref(nf)))
             return w, ins, nf.value

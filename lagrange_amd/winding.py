@@ -29,6 +29,13 @@ def _is_torch(x) -> bool:
     return type(x).__module__.startswith("torch")
 
 
+def _torch_ready(device) -> None:
+    """Wait for torch's current stream on `device`: the engine runs on its own stream, and the
+    input tensors (and the memory of freshly allocated outputs) belong to work queued there."""
+    import torch
+    torch.cuda.current_stream(device).synchronize()
+
+
 def _mesh_arrays(mesh):
     if isinstance(mesh, tuple) and len(mesh) == 2:
         V, F = mesh
@@ -57,7 +64,7 @@ def _create_tuned(n_meshes, V, v_off, F, f_off, points, q_off, leaf_sizes, optio
     if _is_torch(points) and points.is_cuda:
         import torch
         q = points.contiguous().to(torch.float32)
-        torch.cuda.synchronize(q.device)
+        _torch_ready(q.device)
         n, ptr, fl = q.shape[0], _ptr(q), L.FWN_DEVICE_POINTERS
     else:
         q = np.ascontiguousarray(np.asarray(points, dtype=np.float32).reshape(-1, 3))
@@ -130,7 +137,7 @@ class _EngineBase:
         if _is_torch(points) and points.is_cuda:
             import torch
             q = points.contiguous().to(torch.float32)
-            torch.cuda.synchronize(q.device)
+            _torch_ready(q.device)
             n, ptr, fl = q.shape[0], _ptr(q), L.FWN_DEVICE_POINTERS
         else:
             q = np.ascontiguousarray(np.asarray(points, dtype=np.float32).reshape(-1, 3))
@@ -205,7 +212,7 @@ class FastWindingNumber(_EngineBase):
             om = torch.empty(n, dtype=torch.float32, device=q.device)
             # a torch.bool element is one byte holding 0 or 1: the engine's flag bytes, written in place
             ins = torch.empty(n, dtype=torch.bool, device=q.device) if with_inside else None
-            torch.cuda.synchronize(q.device)
+            _torch_ready(q.device)
             L.check(L.load().fwn_solid_angle(self._h(), _ptr(q), n, _ptr(om), _ptr(ins) if ins is not None else None,
                                              L.FWN_DEVICE_POINTERS))
             return (om, ins) if with_inside else om
@@ -237,7 +244,7 @@ class FastWindingNumber(_EngineBase):
             import torch
             ins = torch.empty(n, dtype=torch.bool, device=device)
             om = torch.empty(n, dtype=torch.float32, device=device) if with_omega else None
-            torch.cuda.synchronize(device)
+            _torch_ready(device)
             L.check(L.load().fwn_sign_voxel_box(self._h(), lo_a.ctypes.data, dims_a.ctypes.data, float(voxel_size),
                                                 _ptr(ins), _ptr(om) if om is not None else None,
                                                 L.FWN_DEVICE_POINTERS))
@@ -259,7 +266,7 @@ class FastWindingNumber(_EngineBase):
             n = v.shape[0]
             ins = torch.empty(n, dtype=torch.bool, device=v.device)
             om = torch.empty(n, dtype=torch.float32, device=v.device) if with_omega else None
-            torch.cuda.synchronize(v.device)
+            _torch_ready(v.device)
             L.check(L.load().fwn_sign_voxels(self._h(), _ptr(v), n, float(voxel_size), _ptr(ins),
                                              _ptr(om) if om is not None else None, L.FWN_DEVICE_POINTERS))
             return (ins, om) if with_omega else ins
@@ -281,7 +288,7 @@ class FastWindingNumber(_EngineBase):
             V = vertices.contiguous().to(torch.float32)
             F = facets.contiguous().to(torch.int32)
             out = torch.empty(F.shape[0], dtype=torch.float64, device=V.device)
-            torch.cuda.synchronize(V.device)
+            _torch_ready(V.device)
             L.check(L.load().fwn_orientation_criterion(self._h(), _ptr(V), V.shape[0], _ptr(F), F.shape[0],
                                                        float(epsilon), _ptr(out), L.FWN_DEVICE_POINTERS))
             return out
@@ -303,7 +310,7 @@ class FastWindingNumber(_EngineBase):
         if device is not None:
             import torch
             out = torch.empty((max(n, 1), 3), dtype=torch.float32, device=device)
-            torch.cuda.synchronize(device)
+            _torch_ready(device)
             L.check(L.load().fwn_sample_inside(self._h(), lo.ctypes.data, hi.ctypes.data, n, int(seed), _ptr(out),
                                                C.byref(cnt), L.FWN_DEVICE_POINTERS))
             return out[:cnt.value]
@@ -324,7 +331,7 @@ class FastWindingNumber(_EngineBase):
             d2 = torch.empty(n, dtype=torch.float64, device=q.device)
             fi = torch.empty(n, dtype=torch.int32, device=q.device)
             cp = torch.empty((n, 3), dtype=torch.float64, device=q.device) if with_point else None
-            torch.cuda.synchronize(q.device)
+            _torch_ready(q.device)
             L.check(L.load().fwn_closest_point(self._h(), _ptr(q), n, _ptr(d2), _ptr(fi),
                                                _ptr(cp) if cp is not None else None, L.FWN_DEVICE_POINTERS))
             return (d2, fi, cp) if with_point else (d2, fi)
@@ -352,7 +359,7 @@ class FastWindingNumber(_EngineBase):
             import torch
             q = points.contiguous().to(torch.float32)
             out = torch.empty(q.shape[0], dtype=torch.float64, device=q.device)
-            torch.cuda.synchronize(q.device)
+            _torch_ready(q.device)
             L.check(L.load().fwn_bruteforce_f64(self._h(), _ptr(q), q.shape[0], _ptr(out), L.FWN_DEVICE_POINTERS))
             return out
         q = np.ascontiguousarray(np.asarray(points, dtype=np.float32).reshape(-1, 3))
@@ -421,7 +428,7 @@ class MultiFastWindingNumber(_EngineBase):
             om = torch.empty(n, dtype=torch.float32, device=q.device)
             # a torch.bool element is one byte holding 0 or 1: the engine's flag bytes, written in place
             ins = torch.empty(n, dtype=torch.bool, device=q.device) if with_inside else None
-            torch.cuda.synchronize(q.device)
+            _torch_ready(q.device)
             L.check(L.load().fwn_solid_angle_multi(self._h(), _ptr(q), q_off.ctypes.data, _ptr(om),
                                                    _ptr(ins) if ins is not None else None, L.FWN_DEVICE_POINTERS))
             return (om, ins) if with_inside else om
