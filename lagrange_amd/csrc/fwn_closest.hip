@@ -10,8 +10,9 @@
 // entries as in the winding-number traversal; a lane takes part in a slot while the slot's box is
 // nearer than the lane's best distance so far.  Leaf slots are evaluated before a node's children
 // are pushed (to tighten the bounds early) and children are pushed farthest first, ordered by the
-// wave's smallest box distance, so the nearest is walked next.  Slot boxes come from the build
-// (each slot's vertex box, rounded outward to float).
+// box distance of each slot's first participating lane, so the nearest is walked next.  Inside a
+// leaf, a triangle is evaluated only for lanes whose best distance its vertex box does not
+// exceed.  Slot boxes come from the build (each slot's vertex box, rounded outward to float).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -234,10 +235,16 @@ __device__ __forceinline__ double box_dist2(const float* __restrict__ b, double 
     return r;
 }
 
-__device__ __forceinline__ float wave_min_f(float v)
+// Squared distance from q to the box of the triangle's vertices: a lower bound on the squared
+// distance to the triangle
+__device__ __forceinline__ double tri_box_dist2(const float* __restrict__ tw, double qx, double qy, double qz)
 {
-    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
-    return v;
+    float b[6];
+    for (int d = 0; d < 3; ++d) {
+        b[d] = fminf(fminf(tw[d], tw[3 + d]), tw[6 + d]);
+        b[3 + d] = fmaxf(fmaxf(tw[d], tw[3 + d]), tw[6 + d]);
+    }
+    return box_dist2(b, qx, qy, qz);
 }
 
 template <int NB>
@@ -293,10 +300,14 @@ __global__ __launch_bounds__(64) void k_closest(const uint32_t* __restrict__ slo
                         const int32_t first = (int32_t)nb[s * kSlotWords + FWN_W_FIRST];
                         const int32_t cnt = (int32_t)nb[s * kSlotWords + FWN_W_COUNT];
                         for (int32_t t = first; t < first + cnt; ++t) {
-                            if (go) {
+                            const float* tw = reinterpret_cast<const float*>(tris + (size_t)t * kTriWords);
+                            // a triangle whose box is farther than the lane's best cannot improve it;
+                            // the 1e-12 margin is far above the rounding of the bound and of the
+                            // exact evaluation, so the nearest triangle (and the first of equally
+                            // near ones) is always evaluated and results do not change
+                            if (go && tri_box_dist2(tw, qx, qy, qz) <= best * (1.0 + 1e-12)) {
                                 double ss, tt;
-                                const double d = tri_closest_f64(
-                                    reinterpret_cast<const float*>(tris + (size_t)t * kTriWords), qx, qy, qz, ss, tt);
+                                const double d = tri_closest_f64(tw, qx, qy, qz, ss, tt);
                                 if (d < best) {
                                     best = d;
                                     btri = t;
@@ -315,7 +326,11 @@ __global__ __launch_bounds__(64) void k_closest(const uint32_t* __restrict__ slo
             for (int s = 0; s < kWidth; ++s) {
                 const bool go = s < ns && child[s] >= 0 && act && db[s] < best;
                 pm[s] = __ballot(go);
-                key[s] = pm[s] ? wave_min_f(go ? (float)db[s] : INFINITY) : -1.0f;
+                // ordering key: the box distance seen by the slot's first participating lane (the
+                // wave's points are close together; the order only steers the walk)
+                key[s] = pm[s] ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int((float)db[s]),
+                                                                          (int)__builtin_ctzll(pm[s])))
+                               : -1.0f;
             }
             for (int k = 0; k < kWidth; ++k) {
                 int sel = -1;
