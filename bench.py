@@ -31,13 +31,14 @@ sys.path.insert(0, ROOT)
 # FP32 FLOPs per operation of the traversal kernel, counted from fwn_query.hip
 # (FMA = 2, each rsqrt / atan2 counted as 1):
 #   slot test   : d = p - q (3), |d|^2 (5), compare with the stored beta^2 r2 (1)            = 9
-#   far term    : rsq + 2 powers (3), d.N (5), order 1 in Horner form (14), order 2 in
-#                 Horner form (26), combine + accumulate (6); the constant terms tr M1 and
-#                 C.d are folded into the order-1 and order-2 coefficients by the build      = 54
+#   far term    : rsq + 2 powers (3), d.N (5), dz^2 (1), order 1 in Horner form (13),
+#                 order 2 in Horner form (25), combine + accumulate (6); the constant terms
+#                 tr M1 and C.d are folded into the order-1 and order-2 coefficients by the
+#                 build                                                                     = 53
 #   FP32 tri    : offsets (9), lengths (15), 3 rsq (3), normalise (9), edges (6), cross (9),
 #                 triple product (5), dots (15), denominator (3), atan2 (1), x2 + acc (2)    = 77
 #   FP64 tri    : the same work in FP64 (reported separately, not in the FP32 figure)       = 77
-FLOP_TEST, FLOP_FAR, FLOP_TRI32, FLOP_TRI64 = 9, 54, 77, 77
+FLOP_TEST, FLOP_FAR, FLOP_TRI32, FLOP_TRI64 = 9, 53, 77, 77
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 METRIC = "winding queries/s @1M tris: exact GPU vs surrogate at ≤1e-4 inside-error"

@@ -138,19 +138,20 @@ __device__ __forceinline__ float far_field_acc(const SlotRegs& r, float dx, floa
     const float inv2 = inv * inv;
     const float inv3 = inv2 * inv;
     const float o0 = fmaf(dx, sw(r, FWN_W_NX), fmaf(dy, sw(r, FWN_W_NY), dz * sw(r, FWN_W_NZ)));
-    // A: xx yy zz xy yz zx
+    const float zz = dz * dz; // shared with the slot test's |d|^2
+    // A: xx yy zz xy yz zx;  q1 = A_zz zz + x (A_xx x + A_xy y + A_zx z) + y (A_yy y + A_yz z)
     const float a1 = fmaf(sw(r, FWN_W_A), dx, fmaf(sw(r, FWN_W_A + 3), dy, sw(r, FWN_W_A + 5) * dz));
     const float b1 = fmaf(sw(r, FWN_W_A + 1), dy, sw(r, FWN_W_A + 4) * dz);
-    const float c1 = sw(r, FWN_W_A + 2) * dz;
-    const float q1 = fmaf(dx, a1, fmaf(dy, b1, dz * c1));
-    // B: xxx yyy zzz xxy xxz yyx yyz zzx zzy xyz
-    const float zz = dz * dz;
+    const float q1 = fmaf(dy, b1, fmaf(dx, a1, sw(r, FWN_W_A + 2) * zz));
+    // B: xxx yyy zzz xxy xxz yyx yyz zzx zzy xyz;
+    // q2 = zz (B_zzx x + B_zzy y + B_zzz z) + x (x (B_xxx x + B_xxy y + B_xxz z) + y (B_yyx y + B_xyz z))
+    //      + y y (B_yyy y + B_yyz z)
+    const float L = fmaf(sw(r, FWN_W_B + 7), dx, fmaf(sw(r, FWN_W_B + 8), dy, sw(r, FWN_W_B + 2) * dz));
     const float P = fmaf(sw(r, FWN_W_B), dx, fmaf(sw(r, FWN_W_B + 3), dy, sw(r, FWN_W_B + 4) * dz));
     const float Qv = fmaf(sw(r, FWN_W_B + 5), dy, sw(r, FWN_W_B + 9) * dz);
-    const float X = fmaf(dx, P, fmaf(dy, Qv, sw(r, FWN_W_B + 7) * zz));
-    const float Y = fmaf(dy, fmaf(sw(r, FWN_W_B + 1), dy, sw(r, FWN_W_B + 6) * dz), sw(r, FWN_W_B + 8) * zz);
-    const float Z = sw(r, FWN_W_B + 2) * zz;
-    const float q2 = fmaf(dx, X, fmaf(dy, Y, dz * Z));
+    const float X = fmaf(dy, Qv, dx * P);
+    const float R = fmaf(sw(r, FWN_W_B + 1), dy, sw(r, FWN_W_B + 6) * dz);
+    const float q2 = fmaf(dy * dy, R, fmaf(dx, X, zz * L));
     const float t = fmaf(inv2, fmaf(inv2, q2, q1), o0);
     return fmaf(t, inv3, om);
 }

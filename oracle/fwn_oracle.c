@@ -618,17 +618,16 @@ static inline float far_field_acc(const float* w, float dx, float dy, float dz, 
     const float* A = w + FWN_W_A;
     const float* Bq = w + FWN_W_B;
     const float o0 = fmaf(dx, w[FWN_W_NX], fmaf(dy, w[FWN_W_NY], dz * w[FWN_W_NZ]));
+    const float zz = dz * dz;
     const float a1 = fmaf(A[0], dx, fmaf(A[3], dy, A[5] * dz));
     const float b1 = fmaf(A[1], dy, A[4] * dz);
-    const float c1 = A[2] * dz;
-    const float q1 = fmaf(dx, a1, fmaf(dy, b1, dz * c1));
-    const float zz = dz * dz;
+    const float q1 = fmaf(dy, b1, fmaf(dx, a1, A[2] * zz));
+    const float L = fmaf(Bq[7], dx, fmaf(Bq[8], dy, Bq[2] * dz));
     const float P = fmaf(Bq[0], dx, fmaf(Bq[3], dy, Bq[4] * dz));
     const float Qv = fmaf(Bq[5], dy, Bq[9] * dz);
-    const float X = fmaf(dx, P, fmaf(dy, Qv, Bq[7] * zz));
-    const float Y = fmaf(dy, fmaf(Bq[1], dy, Bq[6] * dz), Bq[8] * zz);
-    const float Z = Bq[2] * zz;
-    const float q2 = fmaf(dx, X, fmaf(dy, Y, dz * Z));
+    const float X = fmaf(dy, Qv, dx * P);
+    const float R = fmaf(Bq[1], dy, Bq[6] * dz);
+    const float q2 = fmaf(dy * dy, R, fmaf(dx, X, zz * L));
     const float t = fmaf(inv2, fmaf(inv2, q2, q1), o0);
     return fmaf(t, inv3, om);
 }
