@@ -417,6 +417,39 @@ def test_small_calls_equal_sorted_calls(ico5):
         np.concatenate([pts[0], rng.uniform(-1.3, 1.3, (5000, 3)).astype(np.float32)]))[:300])
 
 
+def test_repeated_call_shapes_reuse_metadata(ico5):
+    """A call whose shape (meshes, point counts, batch size) matches the previous one reuses the
+    metadata already on the device; any other call uploads its own.  Alternating shapes, small
+    staged calls, host and device arrays and multi-mesh offsets on one engine all give the
+    values of fresh engines bit for bit."""
+    import torch
+    V, F = ico5
+    rng = np.random.default_rng(23)
+    A = rng.uniform(-1.4, 1.4, (5000, 3)).astype(np.float32)
+    B = rng.uniform(-1.4, 1.4, (3000, 3)).astype(np.float32)
+    S = rng.uniform(-1.4, 1.4, (100, 3)).astype(np.float32)
+    fresh = {k: FastWindingNumber((V, F)).solid_angle_batch(x, with_inside=True) for k, x in
+             (("A", A), ("B", B), ("S", S))}
+    e = FastWindingNumber((V, F))
+    for k in ("A", "A", "B", "A", "S", "A", "A", "B", "B", "S", "S", "A"):
+        x = {"A": A, "B": B, "S": S}[k]
+        for dev in (False, True):
+            om, ins = e.solid_angle_batch(torch.from_numpy(x).cuda() if dev else x, with_inside=True)
+            if dev:
+                om, ins = om.cpu().numpy(), ins.cpu().numpy()
+            np.testing.assert_array_equal(om, fresh[k][0])
+            np.testing.assert_array_equal(ins, fresh[k][1])
+    # multi-mesh: the same total count split differently across the meshes
+    meshes = [W.icosphere(6).f32(), (V, F)]
+    mm = MultiFastWindingNumber(meshes)
+    singles = [FastWindingNumber(m) for m in meshes]
+    for split in (2000, 2000, 3000, 2000):
+        q_off = np.array([0, split, A.shape[0]], np.int64)
+        om = mm.solid_angle_batch(points=A, q_off=q_off)
+        np.testing.assert_array_equal(om[:split], singles[0].solid_angle_batch(A[:split]))
+        np.testing.assert_array_equal(om[split:], singles[1].solid_angle_batch(A[split:]))
+
+
 def test_concurrent_point_calls_are_coalesced(ico5):
     """Per-point calls from many threads on one engine (the reference's calling pattern from TBB
     workers) are answered in shared batches: every thread gets the value of a batched call."""
