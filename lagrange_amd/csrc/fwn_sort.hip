@@ -4,13 +4,15 @@
 //
 // One pass = 8 key bits:
 //   hist:    each 256-thread block counts the digits of its 4096-key tile in LDS (the tile's
-//            loads are all issued before the first count)
+//            loads are all issued before the first count; a wave whose keys share one digit
+//            adds its count with one atomic)
 //   rows:    one block per digit scans that digit's per-block counts (digit-major layout) in
 //            chunks of 4,096 staged through LDS
 //   digits:  one block scans the 256 digit totals
 //   scatter: each block loads its tile into registers; each wave ranks its quarter of the tile
-//            in 16 rounds of 64 keys (8 ballots give a key's rank among the wave's equal
-//            digits, wave-private LDS run counters its place in the wave's run), one block step
+//            in 16 rounds of 64 keys (one compare when the wave shares a digit, else 8 ballots,
+//            give a key's rank among the wave's equal digits, wave-private LDS run counters its
+//            place in the wave's run), one block step
 //            turns the waves' runs into stable tile offsets, the tile is placed in LDS in digit
 //            order and written out in coalesced runs.
 #include "fwn_internal.h"
