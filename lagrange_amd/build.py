@@ -110,9 +110,12 @@ def build_cxx(verbose: bool = False) -> str:
     inc = ["-I", os.path.join(CXX_DIR, "include"), "-I", os.path.join(CXX_DIR, "compat"),
            "-I", os.path.join(ROOT, "include")]
     link = ["-L", HERE, "-lfwn", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+    # compat/src/assert.cpp stands in for lagrange::core's assertion handler (a real Lagrange build
+    # links lagrange::core instead)
     cmds = [
         ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-o", CXX_LIB,
-         os.path.join(CXX_DIR, "src", "FastWindingNumber.cpp")] + inc + link,
+         os.path.join(CXX_DIR, "src", "FastWindingNumber.cpp"),
+         os.path.join(CXX_DIR, "compat", "src", "assert.cpp")] + inc + link,
     ]
     os.makedirs(os.path.dirname(CXX_TEST), exist_ok=True)
     cmds.append(["g++", "-std=c++17", "-O2", "-Wall", "-pthread", "-o", CXX_TEST,

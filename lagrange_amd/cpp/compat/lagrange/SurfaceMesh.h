@@ -1,28 +1,21 @@
 // Minimal stand-in for lagrange::SurfaceMesh, used only when the real Lagrange headers are not on
 // the include path.  It offers exactly what the winding engine's constructor consumes
 // (modules/core/include/lagrange/SurfaceMesh.h:72-90 type constraints; get_dimension,
-// is_triangle_mesh; row-major vertex/facet views as modules/core/src/views.cpp:155-175):
-// positions stored row-major (num_vertices x dim), facets with a uniform or mixed vertex count.
+// is_triangle_mesh): positions stored row-major (num_vertices x dim), facets with a uniform or
+// mixed vertex count.  Like the real header (SurfaceMesh.h:14-18) it declares neither
+// lagrange::Error (<lagrange/utils/Error.h>) nor the matrix views (<lagrange/views.h>).
 #pragma once
 
 #include <lagrange/utils/value_ptr.h>
 
-#include <array>
 #include <cstddef>
 #include <cstdint>
+#include <initializer_list>
 #include <stdexcept>
-#include <string>
 #include <type_traits>
 #include <vector>
 
 namespace lagrange {
-
-/// Error raised on invalid input (counterpart of lagrange::Error, a std::runtime_error).
-class Error : public std::runtime_error
-{
-public:
-    using std::runtime_error::runtime_error;
-};
 
 template <typename Scalar_, typename Index_>
 class SurfaceMesh
@@ -55,7 +48,7 @@ public:
     }
     void add_vertex(std::initializer_list<Scalar> p)
     {
-        if (p.size() != m_dim) throw Error("add_vertex: wrong dimension");
+        if (p.size() != m_dim) throw std::invalid_argument("add_vertex: wrong dimension");
         m_pos.insert(m_pos.end(), p.begin(), p.end());
     }
     void add_vertices(Index n, const Scalar* coords) { m_pos.insert(m_pos.end(), coords, coords + n * m_dim); }
@@ -80,31 +73,5 @@ private:
     std::vector<Index> m_corners;
     std::vector<size_t> m_offsets{0};
 };
-
-/// Row-major (rows x cols) read-only view with Eigen-like rows()/cols()/operator()(i, j).
-template <typename T>
-struct RowMajorView
-{
-    const T* data;
-    std::ptrdiff_t nrows, ncols;
-    std::ptrdiff_t rows() const { return nrows; }
-    std::ptrdiff_t cols() const { return ncols; }
-    const T& operator()(std::ptrdiff_t i, std::ptrdiff_t j) const { return data[i * ncols + j]; }
-};
-
-template <typename Scalar, typename Index>
-RowMajorView<Scalar> vertex_view(const SurfaceMesh<Scalar, Index>& mesh)
-{
-    return {mesh.position_data(), (std::ptrdiff_t)mesh.get_num_vertices(), (std::ptrdiff_t)mesh.get_dimension()};
-}
-
-/// Facet view of a mesh whose facets all have the same size (as the reference's facet_view).
-template <typename Scalar, typename Index>
-RowMajorView<Index> facet_view(const SurfaceMesh<Scalar, Index>& mesh)
-{
-    const Index nf = mesh.get_num_facets();
-    const Index k = nf ? mesh.get_facet_size(0) : 3;
-    return {mesh.corner_data(), (std::ptrdiff_t)nf, (std::ptrdiff_t)k};
-}
 
 } // namespace lagrange

@@ -4,6 +4,7 @@
 //
 //   fix_orientation INPUT REFERENCE [OUTPUT] [-e EPSILON=1e-2] [-t THRESHOLD=0.8]
 //                   [--criterion-json PATH]
+#include <lagrange/utils/Error.h>
 #include <lagrange/winding/FastWindingNumber.h>
 
 #include <fwn_io.h>

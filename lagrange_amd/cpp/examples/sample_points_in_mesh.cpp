@@ -3,6 +3,7 @@
 // and compacted on the GPU in one call.
 //
 //   sample_points_in_mesh INPUT [OUTPUT=output.xyz] [-n NUM_SAMPLES=10000] [--seed SEED=0]
+#include <lagrange/utils/Error.h>
 #include <lagrange/winding/FastWindingNumber.h>
 
 #include <fwn_io.h>

@@ -3,7 +3,13 @@
 // Mirrors modules/winding/src/FastWindingNumber.cpp: the constructor asserts a 3D triangle mesh
 // (:91-96), copies positions to float and indices to int (:39-52) and builds the engine
 // (:57); is_inside is solid_angle / (4 pi) > 0.5 in double (:66).  Here the build and every
-// query run on the GPU through fwn_create / fwn_solid_angle*.
+// query run on the GPU through fwn_create / fwn_solid_angle*.  The includes are those a real
+// Lagrange build resolves (SurfaceMeshTypes.h for the X macro, views.h, utils/assert.h,
+// utils/Error.h, as the reference source's :13-18); the compat headers mirror that layout.
+#include <lagrange/SurfaceMeshTypes.h>
+#include <lagrange/utils/Error.h>
+#include <lagrange/utils/assert.h>
+#include <lagrange/views.h>
 #include <lagrange/winding/FastWindingNumber.h>
 
 #include <fwn.h>
@@ -36,8 +42,8 @@ struct FastWindingNumber::Impl
     template <typename VView, typename FView>
     void initialize(const VView& vertices, const FView& facets, const fwn_options* opt = nullptr)
     {
-        if (vertices.cols() != 3) throw Error("Fast winding number engine only supports 3D meshes");
-        if (facets.cols() != 3) throw Error("Fast winding number engine only supports triangle meshes");
+        la_runtime_assert(vertices.cols() == 3);
+        la_runtime_assert(facets.cols() == 3);
         std::vector<float> v(static_cast<size_t>(vertices.rows()) * 3);
         for (std::ptrdiff_t i = 0; i < (std::ptrdiff_t)vertices.rows(); ++i)
             for (int d = 0; d < 3; ++d) v[3 * i + d] = static_cast<float>(vertices(i, d));
@@ -59,8 +65,8 @@ struct FastWindingNumber::Impl
 template <typename Scalar, typename Index>
 FastWindingNumber::FastWindingNumber(const SurfaceMesh<Scalar, Index>& mesh) : m_impl(make_value_ptr<Impl>())
 {
-    if (mesh.get_dimension() != 3) throw Error("Fast winding number engine only supports 3D meshes");
-    if (!mesh.is_triangle_mesh()) throw Error("Fast winding number engine only supports triangle meshes");
+    la_runtime_assert(mesh.get_dimension() == 3, "Fast winding number engine only supports 3D meshes");
+    la_runtime_assert(mesh.is_triangle_mesh(), "Fast winding number engine only supports triangle meshes");
     m_impl->initialize(vertex_view(mesh), facet_view(mesh));
 }
 
@@ -68,8 +74,8 @@ template <typename Scalar, typename Index>
 FastWindingNumber::FastWindingNumber(const SurfaceMesh<Scalar, Index>& mesh, const FastWindingNumberOptions& options)
     : m_impl(make_value_ptr<Impl>())
 {
-    if (mesh.get_dimension() != 3) throw Error("Fast winding number engine only supports 3D meshes");
-    if (!mesh.is_triangle_mesh()) throw Error("Fast winding number engine only supports triangle meshes");
+    la_runtime_assert(mesh.get_dimension() == 3, "Fast winding number engine only supports 3D meshes");
+    la_runtime_assert(mesh.is_triangle_mesh(), "Fast winding number engine only supports triangle meshes");
     fwn_options o;
     fwn_options_default(&o);
     o.accuracy_scale = options.accuracy_scale;
@@ -164,15 +170,12 @@ void FastWindingNumber::closest_point(const std::array<float, 3>* pos, size_t n,
     if (st != FWN_OK) raise("fwn_closest_point", st);
 }
 
-// float/double x uint32/uint64, as LA_SURFACE_MESH_X (modules/core/include/lagrange/SurfaceMeshTypes.h:49-53)
-template FastWindingNumber::FastWindingNumber(const SurfaceMesh<float, uint32_t>&);
-template FastWindingNumber::FastWindingNumber(const SurfaceMesh<double, uint32_t>&);
-template FastWindingNumber::FastWindingNumber(const SurfaceMesh<float, uint64_t>&);
-template FastWindingNumber::FastWindingNumber(const SurfaceMesh<double, uint64_t>&);
-template FastWindingNumber::FastWindingNumber(const SurfaceMesh<float, uint32_t>&, const FastWindingNumberOptions&);
-template FastWindingNumber::FastWindingNumber(const SurfaceMesh<double, uint32_t>&, const FastWindingNumberOptions&);
-template FastWindingNumber::FastWindingNumber(const SurfaceMesh<float, uint64_t>&, const FastWindingNumberOptions&);
-template FastWindingNumber::FastWindingNumber(const SurfaceMesh<double, uint64_t>&, const FastWindingNumberOptions&);
+// float/double x uint32/uint64 (modules/core/include/lagrange/SurfaceMeshTypes.h:49-53)
+#define LA_X_fast_winding_number(_, Scalar, Index)                                                     \
+    template FastWindingNumber::FastWindingNumber(const SurfaceMesh<Scalar, Index>& mesh);            \
+    template FastWindingNumber::FastWindingNumber(const SurfaceMesh<Scalar, Index>& mesh,             \
+                                                  const FastWindingNumberOptions& options);
+LA_SURFACE_MESH_X(fast_winding_number, 0)
 
 } // namespace winding
 } // namespace lagrange

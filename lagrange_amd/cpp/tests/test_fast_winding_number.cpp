@@ -2,6 +2,7 @@
 // Self-contained runner: prints one line per case and returns non-zero on failure.
 // Geometry: the unit cube and octant triangle of the reference's pytest fixtures
 // (modules/conftest.py:29-37, 108-144) and a subdivided octahedron projected to the sphere.
+#include <lagrange/utils/Error.h>
 #include <lagrange/winding/FastWindingNumber.h>
 
 #include <algorithm>
@@ -110,7 +111,10 @@ int main()
                  m.add_triangle(0, 1, 2);
                  FastWindingNumber e(m);
              } catch (const lagrange::Error& ex) {
-                 threw = std::string(ex.what()).find("3D") != std::string::npos;
+                 // la_runtime_assert's message: the condition and the reference's reason text
+                 const std::string w = ex.what();
+                 threw = w.find("Assertion failed") != std::string::npos &&
+                         w.find("Fast winding number engine only supports 3D meshes") != std::string::npos;
              }
              CHECK(threw);
              threw = false;
@@ -120,7 +124,9 @@ int main()
                  m.add_quad(0, 1, 3, 2);
                  FastWindingNumber e(m);
              } catch (const lagrange::Error& ex) {
-                 threw = std::string(ex.what()).find("triangle") != std::string::npos;
+                 const std::string w = ex.what();
+                 threw = w.find("Assertion failed") != std::string::npos &&
+                         w.find("Fast winding number engine only supports triangle meshes") != std::string::npos;
              }
              CHECK(threw);
          }},

@@ -132,6 +132,8 @@ void destroy_engine(Engine* e)
     if (e->h_pipe) (void)hipHostFree(e->h_pipe);
     for (auto& ev : e->pev)
         if (ev) (void)hipEventDestroy(ev);
+    for (auto& ev : e->tev)
+        if (ev) (void)hipEventDestroy(ev);
     if (e->cs_in) (void)hipStreamDestroy(e->cs_in);
     if (e->cs_out) (void)hipStreamDestroy(e->cs_out);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -472,9 +474,13 @@ int fwn_autotune(fwn_engine* h, const float* Q, const int64_t* q_off, int64_t nQ
             while (std::getline(in, line)) {
                 std::istringstream ls(line);
                 std::string k;
-                int w, m;
-                double ms;
-                if (ls >> k >> w >> m >> ms && k == key) {
+                int w = 0, m = -1;
+                double ms = 0.0;
+                if (!(ls >> k >> w >> m >> ms) || k != key) continue;
+                // a line with a shape the engine does not launch (stale or edited file) is ignored
+                const bool shape_ok = (w == 1 || w == 2 || w == 4 || w == 8) &&
+                                      (m == FWN_LAUNCH_GRID || m == FWN_LAUNCH_PERSISTENT);
+                if (shape_ok && std::isfinite(ms) && ms > 0.0) {
                     best_w = w;
                     best_m = m;
                     best = ms;
@@ -501,6 +507,10 @@ int fwn_autotune(fwn_engine* h, const float* Q, const int64_t* q_off, int64_t nQ
                     double t = 1e300;
                     for (int rep = 0; rep < 3; ++rep) { // first run warms caches; keep the best of the rest
                         fwn::run_queries(*e, Q, q_off, nQ, po, nullptr, nullptr, dev);
+                        if (!(e->last_total_ms > 0.0)) {
+                            e->opt = saved;
+                            fwn::fail(FWN_HIP_ERROR, "autotune: the query call reported no device time");
+                        }
                         if (rep > 0) t = std::min(t, e->last_total_ms);
                     }
                     if (t < best) {

@@ -125,6 +125,7 @@ struct Engine {
     DBuf q_pmeta, q_pflags;
 
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<hipEvent_t> tev; // pipelined calls: a (start, end) pair around each chunk's traversal
     double last_traverse_ms = 0.0;
     double last_total_ms = 0.0;
 };

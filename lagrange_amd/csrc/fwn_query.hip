@@ -435,10 +435,11 @@ void launch_traverse(bool stats, int waves, bool persist, int banks, unsigned nb
     case 3: FWN_TRAV(false, 1, true); break;
     case 4: FWN_TRAV(false, 2, false); break;
     case 5: FWN_TRAV(false, 2, true); break;
+    case 8: FWN_TRAV(false, 4, false); break;
     case 9: FWN_TRAV(false, 4, true); break;
     case 16: FWN_TRAV(false, 8, false); break;
     case 17: FWN_TRAV(false, 8, true); break;
-    default: FWN_TRAV(false, 4, false); break;
+    default: fail(FWN_INVALID_ARGUMENT, "unknown traversal launch shape (waves per block 1, 2, 4 or 8)");
     }
 #undef FWN_TRAV
 }
@@ -453,7 +454,7 @@ int resident_blocks(bool persist, int waves, int banks)
     case 5: f = one ? traverse_fn<false, 2, true, 1>() : traverse_fn<false, 2, true, 3>(); break;
     case 9: f = one ? traverse_fn<false, 4, true, 1>() : traverse_fn<false, 4, true, 3>(); break;
     case 17: f = one ? traverse_fn<false, 8, true, 1>() : traverse_fn<false, 8, true, 3>(); break;
-    default: return 1;
+    default: fail(FWN_INVALID_ARGUMENT, "unknown persistent launch shape (waves per block 1, 2, 4 or 8)");
     }
     int nb = 0;
     FWN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, waves * 64, 0));
@@ -520,6 +521,15 @@ void run_queries_pipelined(Engine& e, const float* Q, int64_t nQ, float* omega, 
     if (!e.cs_out) FWN_HIP(hipStreamCreateWithFlags(&e.cs_out, hipStreamNonBlocking));
     if (!e.pev[0])
         for (auto& ev : e.pev) FWN_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    // timing: the call spans the first upload to the last download; each chunk's traversal has
+    // its own event pair (the tuner and fwn_last_timing read these)
+    if (!e.ev[0])
+        for (int k = 0; k < 4; ++k) FWN_HIP(hipEventCreate(&e.ev[k]));
+    while ((int64_t)e.tev.size() < 2 * nch) {
+        hipEvent_t ev = nullptr;
+        FWN_HIP(hipEventCreate(&ev));
+        e.tev.push_back(ev);
+    }
     char* pin = static_cast<char*>(e.h_pipe);
     auto pin_in = [&](int sl) { return reinterpret_cast<float*>(pin + sl * slot_b); };
     auto pin_om = [&](int sl) { return reinterpret_cast<float*>(pin + sl * slot_b + in_b); };
@@ -572,6 +582,7 @@ void run_queries_pipelined(Engine& e, const float* Q, int64_t nQ, float* omega, 
             const int64_t off = k * C, ck = std::min(C, nQ - off);
             if (k >= 2) FWN_HIP(hipEventSynchronize(e.pev[sl])); // chunk k-2's upload has read the buffer
             parallel_copy(pin_in(sl), Q + 3 * off, sizeof(float) * 3 * (size_t)ck);
+            if (k == 0) FWN_HIP(hipEventRecord(e.ev[0], e.cs_in));
             FWN_HIP(hipMemcpyAsync(dIn3 + 3 * off, pin_in(sl), sizeof(float) * 3 * (size_t)ck, hipMemcpyHostToDevice,
                                    e.cs_in));
             FWN_HIP(hipEventRecord(e.pev[sl], e.cs_in));
@@ -596,11 +607,13 @@ void run_queries_pipelined(Engine& e, const float* Q, int64_t nQ, float* omega, 
             uint32_t* dnitems = reinterpret_cast<uint32_t*>(dflag + 1);
             uint32_t* dwork = reinterpret_cast<uint32_t*>(dflag + 2);
             hipLaunchKernelGGL(k_zero3, dim3(1), dim3(1), 0, s, dflag);
+            FWN_HIP(hipEventRecord(e.tev[2 * k], s));
             launch_traverse(false, waves, persist, banks, (unsigned)nblk, s, e.slots.as<uint32_t>(), e.width,
                             e.tris.as<uint32_t>(), e.roots.as<int32_t>(), e.mesh_info.as<float>(), dQ, vp, d_qoff,
                             d_boff, d_order, 1, nb, 64, beta2, dOm + off, dIn ? dIn + off : nullptr, (int32_t*)nullptr,
                             e.q_items.as<uint32_t>(), dnitems, cap, dflag, dwork);
             FWN_LAUNCH_CHECK("k_traverse");
+            FWN_HIP(hipEventRecord(e.tev[2 * k + 1], s));
             hipLaunchKernelGGL(k_near64, dim3(2048), dim3(256), 0, s, e.q_items.as<uint32_t>(), dnitems, cap,
                                e.tris.as<uint32_t>(), dQ, vp, e.q_acc.as<unsigned long long>());
             FWN_LAUNCH_CHECK("k_near64");
@@ -615,6 +628,7 @@ void run_queries_pipelined(Engine& e, const float* Q, int64_t nQ, float* omega, 
             if (inside)
                 FWN_HIP(hipMemcpyAsync(pin_i8(sl), dIn + off, (size_t)ck, hipMemcpyDeviceToHost, e.cs_out));
             FWN_HIP(hipEventRecord(e.pev[4 + sl], e.cs_out));
+            if (k == nch - 1) FWN_HIP(hipEventRecord(e.ev[3], e.cs_out));
         }
         if (k >= 1) { // chunk k-1 to the caller's arrays
             const int64_t j = k - 1;
@@ -630,17 +644,28 @@ void run_queries_pipelined(Engine& e, const float* Q, int64_t nQ, float* omega, 
     FWN_HIP(hipStreamSynchronize(s));
     FWN_HIP(hipStreamSynchronize(e.cs_in));
     FWN_HIP(hipStreamSynchronize(e.cs_out));
+    float t_all = 0.0f;
+    double t_tr = 0.0;
+    FWN_HIP(hipEventElapsedTime(&t_all, e.ev[0], e.ev[3]));
+    for (int64_t k = 0; k < nch; ++k) {
+        float t = 0.0f;
+        FWN_HIP(hipEventElapsedTime(&t, e.tev[2 * k], e.tev[2 * k + 1]));
+        t_tr += t;
+    }
+    e.last_traverse_ms = t_tr / (double)nch; // average over the chunks' launches
+    e.last_total_ms = t_all;
     for (int64_t k = 0; k < nch; ++k) {
         if (fl[4 * k]) fail(FWN_LIMIT_EXCEEDED, "traversal stack overflow");
         const int64_t need = (int64_t)(uint32_t)fl[4 * k + 1] * kItemChunk;
         if (need > (int64_t)cap) { // the chunk's FP64 work list overflowed its capacity in the
                                    // pipeline (a redo may have grown it since): answer it again
             const int64_t off = k * C, ck = std::min(C, nQ - off);
+            const double tr = e.last_traverse_ms, tot = e.last_total_ms;
             run_queries(e, Q + 3 * off, nullptr, ck, omega + off, inside ? inside + off : nullptr, nullptr, false);
+            e.last_traverse_ms = tr;
+            e.last_total_ms = tot + e.last_total_ms; // the redo is part of this call's device time
         }
     }
-    e.last_traverse_ms = 0.0;
-    e.last_total_ms = 0.0;
 }
 
 } // namespace

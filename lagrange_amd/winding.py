@@ -56,11 +56,35 @@ def _ptr(a) -> int:
     return a.ctypes.data if isinstance(a, np.ndarray) else int(a.data_ptr())
 
 
+def _points_rows(points) -> int:
+    if _is_torch(points):
+        return int(points.reshape(-1, 3).shape[0])
+    return int(np.asarray(points).reshape(-1, 3).shape[0])
+
+
+def _check_q_off(q_off, n_meshes: int, n_points: int) -> np.ndarray:
+    """Per-mesh point offsets as the C ABI reads them: n_meshes + 1 int64 entries, starting at 0,
+    non-decreasing, ending at the number of points.  The library takes the point count from the
+    last entry, so a wrong array would make it read past the caller's buffers: reject it here."""
+    qo = np.ascontiguousarray(np.asarray(q_off).reshape(-1), dtype=np.int64)
+    if qo.shape[0] != n_meshes + 1:
+        raise LagrangeError(f"q_off has {qo.shape[0]} entries; {n_meshes + 1} (meshes + 1) are required")
+    if qo[0] != 0:
+        raise LagrangeError("q_off[0] must be 0")
+    if np.any(np.diff(qo) < 0):
+        raise LagrangeError("q_off must be non-decreasing")
+    if qo[-1] != n_points:
+        raise LagrangeError(f"q_off[-1] = {int(qo[-1])} differs from the number of points ({n_points})")
+    return qo
+
+
 def _create_tuned(n_meshes, V, v_off, F, f_off, points, q_off, leaf_sizes, options, opts):
     """Shared body of the tuned constructors: (handle, chosen leaf size, query ms)."""
+    if q_off is None and n_meshes != 1:
+        raise LagrangeError("per-mesh point offsets are required for a multi-mesh engine")
+    qo = None if q_off is None else _check_q_off(q_off, n_meshes, _points_rows(points))
     o = options if options is not None else L.default_options(**opts)
     leaves = np.ascontiguousarray(np.asarray(leaf_sizes, dtype=np.int32).reshape(-1))
-    qo = None if q_off is None else np.ascontiguousarray(q_off, dtype=np.int64)
     if _is_torch(points) and points.is_cuda:
         import torch
         q = points.contiguous().to(torch.float32)
@@ -133,7 +157,10 @@ class _EngineBase:
     def autotune(self, points, q_off=None, cache_path: str | None = None):
         """Time every launch shape on these points and keep the fastest: (waves, mode, ms)."""
         w, m, ms = C.c_int32(), C.c_int32(), C.c_double()
-        qo = None if q_off is None else np.ascontiguousarray(q_off, dtype=np.int64)
+        n_meshes = getattr(self, "num_meshes", 1)
+        if q_off is None and n_meshes != 1:
+            raise LagrangeError("per-mesh point offsets are required for a multi-mesh engine")
+        qo = None if q_off is None else _check_q_off(q_off, n_meshes, _points_rows(points))
         if _is_torch(points) and points.is_cuda:
             import torch
             q = points.contiguous().to(torch.float32)
@@ -420,7 +447,9 @@ class MultiFastWindingNumber(_EngineBase):
                 points = torch.cat([p.reshape(-1, 3) for p in points_per_mesh])
             else:
                 points = np.concatenate([np.asarray(p, np.float32).reshape(-1, 3) for p in points_per_mesh])
-        q_off = np.ascontiguousarray(q_off, dtype=np.int64)
+        elif q_off is None:
+            raise LagrangeError("q_off is required with concatenated points")
+        q_off = _check_q_off(q_off, self.num_meshes, _points_rows(points))
         if _is_torch(points) and points.is_cuda:
             import torch
             q = points.contiguous().to(torch.float32)
@@ -441,7 +470,7 @@ class MultiFastWindingNumber(_EngineBase):
 
     def query_stats(self, points, q_off):
         q = np.ascontiguousarray(np.asarray(points, dtype=np.float32).reshape(-1, 3))
-        q_off = np.ascontiguousarray(q_off, dtype=np.int64)
+        q_off = _check_q_off(q_off, self.num_meshes, q.shape[0])
         om = np.empty(q.shape[0], np.float32)
         st = np.empty((q.shape[0], 4), np.int32)
         L.check(L.load().fwn_query_stats(self._h(), q.ctypes.data, q_off.ctypes.data, q.shape[0], om.ctypes.data,

@@ -480,3 +480,28 @@ def test_pipelined_host_calls_equal_device_calls():
         om_d, in_d = e.solid_angle_batch(torch.from_numpy(q).cuda(), with_inside=True)
         np.testing.assert_array_equal(om_h, om_d.cpu().numpy())
         np.testing.assert_array_equal(in_h, in_d.cpu().numpy())
+
+
+def test_pipelined_calls_report_device_time_and_tune(tmp_path):
+    """Host-array calls above one chunk run pipelined; they report their device time (first
+    upload to last download, and the average traversal per chunk), so the tuner times them."""
+    V, F = W.icosphere(24).f32()
+    e = FastWindingNumber((V, F))
+    q = np.random.default_rng(9).uniform(-1.4, 1.4, (3_000_000, 3)).astype(np.float32)
+    e.solid_angle_batch(q)
+    tr, tot = e.last_timing()
+    assert tot > 0 and 0 < tr < tot
+    w, mode, ms = e.autotune(q)
+    assert w in (1, 2, 4, 8) and mode in (0, 1) and ms > 0.5 * tot
+    single = FastWindingNumber.tuned((V, F), q, (1, 8))
+    assert single.tuning["leaf_max"] in (1, 8) and single.tuning["query_ms"] > 0
+    # cache lines naming shapes the engine does not launch are ignored, not used
+    cache = tmp_path / "tune.txt"
+    small = q[:20000]
+    e2 = FastWindingNumber((V, F))
+    w2, m2, _ = e2.autotune(small, cache_path=str(cache))
+    key = open(cache).read().split()[0]
+    with open(cache, "a") as f:
+        f.write(f"{key} 3 1 0.001\n{key} 4 7 0.001\n{key} 8 0 -1\n{key} 8 0 nan\n")
+    e3 = FastWindingNumber((V, F))
+    assert e3.autotune(small, cache_path=str(cache))[:2] == (w2, m2)

@@ -115,3 +115,41 @@ def test_bench_refuses_multi_gpu():
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_q_off_is_checked_before_any_library_call(monkeypatch):
+    """Every wrapper that takes per-mesh point offsets validates them on the host: the library
+    reads the point count from q_off[n_meshes], so a short or inconsistent array would make it
+    read past the caller's buffers."""
+    from lagrange_amd import winding as Wd
+
+    def no_library():
+        raise AssertionError("the library was called before the offsets were checked")
+
+    monkeypatch.setattr(L, "load", no_library)
+    e = Wd.MultiFastWindingNumber.__new__(Wd.MultiFastWindingNumber)
+    e._handle = C.c_void_p(1)  # never dereferenced: every call below must fail first
+    e.num_meshes = 3
+    pts = np.zeros((10, 3), np.float32)
+    bad = {
+        "short": np.array([0, 4, 10]),
+        "long": np.array([0, 4, 6, 8, 10]),
+        "nonzero start": np.array([1, 4, 6, 10]),
+        "decreasing": np.array([0, 6, 4, 10]),
+        "too large": np.array([0, 4, 6, 11]),
+        "too small": np.array([0, 4, 6, 9]),
+    }
+    for name, qo in bad.items():
+        with pytest.raises(Wd.LagrangeError):
+            e.solid_angle_batch(points=pts, q_off=qo)
+        with pytest.raises(Wd.LagrangeError):
+            e.query_stats(pts, qo)
+        with pytest.raises(Wd.LagrangeError):
+            e.autotune(pts, q_off=qo)
+        with pytest.raises(Wd.LagrangeError):
+            Wd._create_tuned(3, None, None, None, None, pts, qo, (1,), None, {})
+    with pytest.raises(Wd.LagrangeError):
+        e.solid_angle_batch(points=pts)  # concatenated points without offsets
+    with pytest.raises(Wd.LagrangeError):
+        e.autotune(pts)  # a multi-mesh engine needs offsets
+    assert np.array_equal(Wd._check_q_off([0, 4, 6, 10], 3, 10), [0, 4, 6, 10])
