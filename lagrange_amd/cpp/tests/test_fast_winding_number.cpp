@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <functional>
 #include <map>
+#include <random>
 #include <string>
 #include <thread>
 #include <utility>
@@ -164,6 +165,25 @@ int main()
                  if (r < 0.9) CHECK(in[i]);
                  if (r > 1.1) CHECK(!in[i]);
              }
+         }},
+        {"reference sampling loop: serial per-point is_inside equals one batched call", [] {
+             // modules/winding/examples/sample_points_in_mesh.cpp:60-76 and the benchmark of
+             // modules/winding/tests/test_fast_winding_number.cpp:99-107: std::mt19937 points
+             // uniform in the bounding box, one is_inside call each
+             FastWindingNumber e(sphere(16));
+             std::uniform_real_distribution<float> px(-1.f, 1.f), py(-1.f, 1.f), pz(-1.f, 1.f);
+             std::mt19937 gen;
+             std::vector<std::array<float, 3>> pts(2000);
+             for (auto& p : pts) p = {px(gen), py(gen), pz(gen)};
+             std::vector<std::array<float, 3>> serial, batched;
+             for (const auto& p : pts)
+                 if (e.is_inside(p)) serial.push_back(p);
+             std::vector<char> in(pts.size());
+             e.is_inside(pts.data(), pts.size(), reinterpret_cast<bool*>(in.data()));
+             for (size_t k = 0; k < pts.size(); ++k)
+                 if (in[k]) batched.push_back(pts[k]);
+             CHECK(serial == batched);
+             CHECK(serial.size() > 600 && serial.size() < 1400);
          }},
         {"voxel signs: box and list equal per-point is_inside of the voxel centres", [] {
              // mesh_to_volume's interior test (modules/volume/src/mesh_to_volume.cpp:175-183):

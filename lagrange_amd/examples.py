@@ -7,12 +7,14 @@ vertices swapped).  Defaults epsilon 1e-2, threshold 0.8 (:40-41).
 
 ``sample_points_in_mesh`` (modules/winding/examples/sample_points_in_mesh.cpp): uniform samples in
 the mesh's bounding box, kept when inside, written as an .xyz file.  Default 10,000 samples
-(:36).
+(:36).  The samples are the reference's own sequence (std::mt19937 with its default seed, one
+std::uniform_real_distribution<float> per axis, drawn x, y, z per sample, :60-71), answered by one
+batched is_inside call; the kept points and the file text equal the reference program's.
 
 Command line (mirrors the reference programs' arguments):
   python -m lagrange_amd.examples fix_orientation INPUT REFERENCE [OUTPUT] [-e EPS] [-t THRESH]
                                    [--criterion-json PATH]
-  python -m lagrange_amd.examples sample_points_in_mesh INPUT [OUTPUT.xyz] [-n N] [--seed S]
+  python -m lagrange_amd.examples sample_points_in_mesh INPUT [OUTPUT.xyz] [-n N]
 """
 from __future__ import annotations
 
@@ -38,10 +40,60 @@ def fix_orientation(vertices, facets, reference, epsilon: float = 1e-2, threshol
     return F, crit, counts
 
 
-def sample_points_in_mesh(mesh, num_samples: int = 10000, seed: int = 0):
-    """Samples uniform in the mesh's bounding box that are inside the mesh, in sample order."""
+MT19937_DEFAULT_SEED = 5489  # std::mt19937's default seed (std::mersenne_twister_engine::default_seed)
+
+
+def mt19937_box_samples(lo, hi, num_samples: int, seed: int = MT19937_DEFAULT_SEED) -> np.ndarray:
+    """The points the reference draws (sample_points_in_mesh.cpp:60-71): std::mt19937 seeded as a
+    default-constructed engine, and per axis a std::uniform_real_distribution<float>(lo, hi) whose
+    value is float(u * (hi - lo)) + lo, each operation rounded to float, with
+    u = generate_canonical<float, 24>: one 32-bit draw x, u = float(x) / 2^32, and u = 1 - 2^-24
+    when float(x) rounds up to 2^32 (libstdc++).  Draws go x, y, z per sample."""
+    lo = np.asarray(lo, np.float32).reshape(3)
+    hi = np.asarray(hi, np.float32).reshape(3)
+    n = int(num_samples)
+    bg = np.random.MT19937()
+    bg._legacy_seeding(int(seed))  # init_genrand(seed): the std::mt19937(seed) state
+    x = bg.random_raw(3 * n).astype(np.uint32).reshape(n, 3)
+    u = x.astype(np.float32) / np.float32(4294967296.0)
+    u[u >= np.float32(1)] = np.nextafter(np.float32(1), np.float32(0))
+    return (u * (hi - lo)) + lo  # float32 multiply, then float32 add (two roundings, as the source reads)
+
+
+def format_float(v) -> str:
+    """A float32 as fmt's "{}" writes it: the shortest round-trip digits, exponent form
+    (d[.ddd]e+XX) when the decimal exponent is < -4 or >= 7, fixed form otherwise with no
+    trailing ".0" (lagrange_amd/cpp/examples/format_float.h is the C++ twin)."""
+    v = np.float32(v)
+    if not np.isfinite(v):
+        return "nan" if np.isnan(v) else ("inf" if v > 0 else "-inf")
+    s = np.format_float_scientific(v, unique=True, trim="-", exp_digits=2)
+    neg = s.startswith("-")
+    s = s.lstrip("-")
+    mant, ex = s.split("e")
+    x = int(ex)
+    digits = mant.replace(".", "")
+    if x < -4 or x >= 7:
+        out = digits[0] + ("." + digits[1:] if len(digits) > 1 else "") + ("e-" if x < 0 else "e+") + f"{abs(x):02d}"
+    elif x >= 0:
+        out = digits + "0" * (x + 1 - len(digits)) if len(digits) <= x + 1 else digits[:x + 1] + "." + digits[x + 1:]
+    else:
+        out = "0." + "0" * (-x - 1) + digits
+    return "-" + out if neg else out
+
+
+def xyz_text(points) -> str:
+    """Lines "x y z" as the reference writes them (fmt::join(p, " "), :80-83)."""
+    return "".join(f"{format_float(p[0])} {format_float(p[1])} {format_float(p[2])}\n" for p in points)
+
+
+def sample_points_in_mesh(mesh, num_samples: int = 10000):
+    """The reference example's kept points, in sample order: its mt19937 sequence over the float
+    bounding box, tested with one batched is_inside call."""
     engine = mesh if isinstance(mesh, FastWindingNumber) else FastWindingNumber(mesh)
-    return engine.sample_inside(num_samples, seed)
+    lo, hi = engine.bbox
+    pts = mt19937_box_samples(lo, hi, num_samples)
+    return pts[engine.is_inside_batch(pts)]
 
 
 def _load_triangles(path):
@@ -63,7 +115,6 @@ def main(argv=None) -> int:
     sp.add_argument("input")
     sp.add_argument("output", nargs="?", default="output.xyz")
     sp.add_argument("-n", "--num-samples", type=int, default=10000)
-    sp.add_argument("--seed", type=int, default=0)
     a = ap.parse_args(argv)
     if a.cmd == "fix_orientation":
         V, F = _load_triangles(a.input)
@@ -79,8 +130,9 @@ def main(argv=None) -> int:
     V, F = _load_triangles(a.input)
     if not a.output.endswith(".xyz"):
         print(f"Output file extension should be .xyz. '{a.output}' was given.", file=sys.stderr)
-    pts = sample_points_in_mesh((V, F), a.num_samples, a.seed)
-    np.savetxt(a.output, pts, fmt="%.9g")
+    pts = sample_points_in_mesh((V, F), a.num_samples)
+    with open(a.output, "w") as f:
+        f.write(xyz_text(pts))
     print(f"{pts.shape[0]} of {a.num_samples} samples inside")
     return 0
 

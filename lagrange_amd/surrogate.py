@@ -122,7 +122,17 @@ class Surrogate:
         return w, ins.astype(bool), nf.value
 
     def predict(self, points):
-        """(network prediction, fallback flag) per point, host arrays."""
+        """(network prediction, fallback flag) per point: numpy in -> numpy out, torch device
+        tensor in -> device tensors out."""
+        if _is_torch(points) and points.is_cuda:
+            import torch
+            q = points.contiguous().to(torch.float32)
+            p = torch.empty(q.shape[0], dtype=torch.float32, device=q.device)
+            fb = torch.empty(q.shape[0], dtype=torch.bool, device=q.device)
+            _torch_ready(q.device)
+            L.check(_bind().fwn_surrogate_predict(self._h, _ptr(q), q.shape[0], _ptr(p), _ptr(fb),
+                                                  L.FWN_DEVICE_POINTERS))
+            return p, fb
         q = np.ascontiguousarray(np.asarray(points, np.float32).reshape(-1, 3))
         p = np.empty(q.shape[0], np.float32)
         fb = np.empty(q.shape[0], np.uint8)

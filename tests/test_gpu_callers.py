@@ -8,13 +8,17 @@ fix_orientation (modules/winding/examples/fix_orientation.cpp:86-115):
     +1 for flipped ones, so fixing a mesh with flipped facets restores it exactly;
   * the CPU restatement of the reference engine (oracle) gives the same criterion within twice
     the order-2 error bound (2 x 4e-3 in w).
-sample_points_in_mesh (modules/winding/examples/sample_points_in_mesh.cpp:59-76):
-  * bit-exact: the kept points equal host-regenerated samples (same counter-based generator)
-    filtered by the engine's own is_inside, in sample order;
-  * the kept fraction matches the volume ratio mesh / box (divergence theorem) within 4 sigma.
+sample_points_in_mesh (modules/winding/examples/sample_points_in_mesh.cpp:59-83):
+  * reference sequence: the C++ example (std::mt19937 + std::uniform_real_distribution<float>)
+    and the Python restatement write the same .xyz text, and their kept points equal the
+    reference's serial loop of per-point is_inside calls bit for bit;
+  * the batched counter-based sampler (fwn_sample_inside): kept points equal host-regenerated
+    samples filtered by the engine's own is_inside, in sample order, and the kept fraction
+    matches the volume ratio mesh / box (divergence theorem) within 4 sigma.
 """
 import os
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -27,7 +31,8 @@ from lagrange_amd.winding import FastWindingNumber
 
 pytestmark = pytest.mark.gpu
 
-CPP_BUILD = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lagrange_amd", "cpp", "build")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CPP_BUILD = os.path.join(ROOT, "lagrange_amd", "cpp", "build")
 
 
 def host_probes(V, F, eps):
@@ -148,11 +153,30 @@ def test_cpp_example_programs(tmp_path, ico5):
     import json
     crit = np.array(json.load(open(tmp_path / "c.json"))["criterions"])
     np.testing.assert_array_equal(crit, FastWindingNumber((V, F)).orientation_criterion(V, Fin, 1e-2))
+    # sample_points_in_mesh: the reference's mt19937 sequence, batched is_inside, fmt "{}" text
     exe = os.path.join(CPP_BUILD, "sample_points_in_mesh")
-    r = subprocess.run([exe, str(tmp_path / "ref.obj"), str(tmp_path / "s.xyz"), "-n", "50000", "--seed", "3"],
+    r = subprocess.run([exe, str(tmp_path / "ref.obj"), str(tmp_path / "s.xyz"), "-n", "50000"],
                        capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0
-    xyz = np.loadtxt(tmp_path / "s.xyz", dtype=np.float32).reshape(-1, 3)
-    want = FastWindingNumber((V, F)).sample_inside(50000, 3)
-    np.testing.assert_array_equal(xyz, want)
+    want = EX.sample_points_in_mesh(FastWindingNumber((V, F)), 50000)
+    assert r.stdout.strip() == f"{want.shape[0]} of 50000 samples inside"
+    assert open(tmp_path / "s.xyz").read() == EX.xyz_text(want)
+    r = subprocess.run([sys.executable, "-m", "lagrange_amd.examples", "sample_points_in_mesh",
+                        str(tmp_path / "ref.obj"), str(tmp_path / "p.xyz"), "-n", "50000"],
+                       capture_output=True, text=True, timeout=300, cwd=ROOT)
+    assert r.returncode == 0, r.stderr
+    assert open(tmp_path / "p.xyz").read() == open(tmp_path / "s.xyz").read()
+
+
+def test_reference_sampling_equals_serial_per_point_loop(ico5):
+    """The kept points of the batched sampler equal the reference's loop (one is_inside call per
+    sample, in order, modules/winding/examples/sample_points_in_mesh.cpp:69-76) bit for bit."""
+    V, F = ico5
+    e = FastWindingNumber((V, F))
+    n = 10000
+    pts = EX.mt19937_box_samples(*e.bbox, n)
+    serial = np.array([p for p in pts if e.is_inside(p)], np.float32).reshape(-1, 3)
+    kept = EX.sample_points_in_mesh(e, n)
+    np.testing.assert_array_equal(kept.view(np.uint32), serial.view(np.uint32))
+    assert 0.3 < kept.shape[0] / n < 0.7
