@@ -428,7 +428,54 @@ def bench_io(mesh):
     return out
 
 
-def bench_surrogate(mesh, steps: int = 3):
+def bench_point_calls(mesh, n: int = 10000, threads: int = 16, runs: int = 3):
+    """The reference's per-point benchmark (modules/winding/tests/test_fast_winding_number.cpp:85-110)
+    through the C++ drop-in class (lagrange_amd/cpp/bench/bench_point_calls.cpp): n serial
+    is_inside calls on std::mt19937 points over the mesh's float bounding box, the same from
+    `threads` callers at once, the n points in one batched call, and batched calls of k points.
+    Beside it the CPU restatement answering the same n points one at a time on one thread (the
+    reference's own calling pattern on the host)."""
+    import subprocess
+    import tempfile
+
+    import numpy as np
+
+    from lagrange_amd import build as B
+    from lagrange_amd import io as mio
+    from lagrange_amd.examples import mt19937_box_samples
+    from oracle import oracle as O
+
+    V = mesh.vertices.astype(np.float64)
+    F = mesh.facets.astype(np.int32)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "mesh.ply")
+        mio.save_mesh(path, V, F)
+        r = subprocess.run([B.CXX_BENCH, path, "-n", str(n), "-t", str(threads), "-r", str(runs)],
+                           capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        return {"error": (r.stdout + r.stderr)[-800:]}
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    Vf, Ff = mesh.f32()
+    pts = mt19937_box_samples(Vf.min(0), Vf.max(0), n)
+    R = O.Restated(Vf, Ff)
+    R.solid_angle(pts[:256], nthreads=1)
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        om = R.solid_angle(pts, nthreads=1)
+        ts.append(time.perf_counter() - t0)
+    dt = sorted(ts)[1]
+    out["cpu_restated_one_thread"] = {"us_per_point": 1e6 * dt / n, "points_per_s": n / dt,
+                                      "num_insides": int(O.inside_from_omega(om).sum())}
+    bk = out.get("batched_call_us", {})
+    cross = [int(k) for k, us in bk.items() if us < int(k) * out["serial_us_per_call"]]
+    out["crossover_points"] = min(cross) if cross else None
+    out["workload"] = (f"{F.shape[0]} triangles; {n} points from std::mt19937 (default seed), uniform in the "
+                       f"float bbox; {threads} threads x {n} calls for the threaded rate")
+    return out
+
+
+def bench_surrogate(mesh, steps: int = 3, train_steps: int = 0):
     """Surrogate on the ~1M-triangle mesh: 16 x 2^19 hash grid (2 FP16 features) + 4 x 128 ReLU MLP
     (BF16 MFMA), trained on 20M points labelled by the exact engine (half uniform, half near the
     surface; points inside the distance bound carry no loss); 67,108,864 uniform points in the box answered by surrogate + exact fallback
@@ -442,7 +489,9 @@ def bench_surrogate(mesh, steps: int = 3):
 
     V, F = mesh.f32()
     e = FastWindingNumber((V, F))
-    s = Surrogate(e)
+    # train_steps > 0 shortens training (profiling runs of the inference kernels, whose work per
+    # point does not depend on the weights); 0 keeps the default 3,000 Adam steps
+    s = Surrogate(e, steps=train_steps) if train_steps > 0 else Surrogate(e)
     info = s.train()
     lo, hi = V.min(0), V.max(0)
     n = 1 << 26
@@ -505,9 +554,11 @@ def main():
     ap.add_argument("--no-tuned", action="store_true",
                     help="skip the tuner's engine beside the headline (keeps a profile to the default engine)")
     ap.add_argument("--cpu-stride", type=int, default=1)
-    ap.add_argument("--extras", default="batched,near,voxel,examples,closest,io,accuracy,surrogate",
+    ap.add_argument("--extras", default="batched,near,voxel,examples,closest,io,points,accuracy,surrogate",
                     help="comma list of extra workloads reported beside the headline: batched, near, voxel, "
                          "examples, closest, io, accuracy, surrogate")
+    ap.add_argument("--surrogate-train-steps", type=int, default=0,
+                    help="Adam steps of the surrogate leg (0: the default 3,000; fewer for profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (scale for rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -667,6 +718,8 @@ def main():
         line["closest"] = bench_closest(mesh, q)
     if "io" in extras:
         line["io"] = bench_io(mesh)
+    if "points" in extras:
+        line["point_calls"] = bench_point_calls(mesh)
     acc = None
     if "accuracy" in extras:
         line["accuracy_scale_sweep"], idx_a, wb_a = bench_accuracy(V, F, q, qd)
@@ -675,10 +728,13 @@ def main():
     if "surrogate" in extras:
         del qd, om, ins
         torch.cuda.empty_cache()
-        line["surrogate"] = bench_surrogate(mesh)
+        line["surrogate"] = bench_surrogate(mesh, train_steps=args.surrogate_train_steps)
     if not args.no_cpu:
+        # every core this process may run on; a GPU box gives one GPU's process its CPU share
+        # through OMP_NUM_THREADS (16 there), which bounds the OpenMP pool the same way
         threads = len(os.sched_getaffinity(0))
-        threads = min(threads, 16)
+        if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+            threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
         line["cpu_baseline"] = cpu_baseline(V, F, q, args.cpu_stride, threads, acc)
         line["speedup_vs_cpu"] = value / line["cpu_baseline"]["value"]
         hit = line.get("accuracy_scale_sweep", {}).get("at_1e-3")

@@ -101,41 +101,51 @@ enum { FWN_LAUNCH_GRID = 0, FWN_LAUNCH_PERSISTENT = 1 };
 
 /* ---- exported tree format (fwn_export_tree) ------------------------------------------------
  * The engine stores, per mesh, a tree of nodes with tree_width (<= FWN_WIDTH) slots each; node
- * n's slot s is at word (n * tree_width + s) * FWN_SLOT_WORDS.  Each slot is 32 32-bit words.
- * Words 0..26 are float, 27..31 int32:
- *   0-2  p     area-weighted centroid of the slot's triangles
- *   3    r2    squared radius bound: every point of the slot's triangles is within sqrt(r2) of p
- *   4-6  N     sum of area-weighted normals
- *   7    tr    trace of the first moment  M1_ji = sum_t A_t (c_t-p)_j n_t,i
- *   8-13 -3 * (M1xx, M1yy, M1zz, M1xy+M1yx, M1yz+M1zy, M1zx+M1xz), plus tr on xx, yy, zz
- *   14-23 7.5 * (S_xxx, S_yyy, S_zzz, 3S_xxy, 3S_xxz, 3S_yyx, 3S_yyz, 3S_zzx, 3S_zzy, 6S_xyz)
- *        where S is the fully symmetrised second moment  int (x-p)_j (x-p)_k n_i dA,
- *        plus C_x on xxx, yyx, zzx, C_y on yyy, xxy, zzy and C_z on zzz, xxz, yyz
- *   24-26 C = -4.5 * t,  t_k = sum_i S_iik
- * With d = p - q the far field is |d|^-3 (d.N + |d|^-2 (q1 + |d|^-2 q2)), q1 the quadratic form
- * of words 8-13 and q2 the cubic of words 14-23: tr and C.d are folded into them through
- * |d|^2 / |d|^2, and words 7 and 24-26 are not read by the evaluation.
- *   27   child node index (internal slot) or -1 (leaf slot)
- *   28   first triangle (position in the engine's sorted triangle array)
- *   29   triangle count covered by the slot
- *   30   number of occupied slots of the child node (internal slots), 0 for leaves
- *   31   beta^2 r2 (float): the far-field threshold for the engine's accuracy scale
- * Triangles are stored in tree order, FWN_TRI_WORDS words each: v0, v1, v2 (9 floats),
- * original facet index (int32), 2 reserved words.
+ * n's slot s is at word (n * tree_width + s) * FWN_SLOT_WORDS, and a node's occupied slots come
+ * first.  Each slot is 24 32-bit words (96 bytes: one 16-word and one 8-word scalar load).
+ * Words 0..22 are float, word 23 is the slot's link:
+ *   0-2   p     area-weighted centroid of the slot's triangles
+ *   3     thr   beta^2 r2 in float: r2 is the squared distance from p to the farthest corner of
+ *               the slot's box (rounded up, so every point of the slot's triangles lies within
+ *               sqrt(r2) of p) and beta the engine's accuracy scale; the slot takes the far field
+ *               for a query q when |p - q|^2 > thr
+ *   4-6   N     sum of area-weighted normals
+ *   7-12  A     -3 * (M1xx, M1yy, M1zz, M1xy+M1yx, M1yz+M1zy, M1zx+M1xz), plus tr M1 on xx, yy,
+ *               zz, where M1_ji = sum_t A_t (c_t-p)_j n_t,i
+ *   13-22 B     7.5 * (S_xxx, S_yyy, S_zzz, 3S_xxy, 3S_xxz, 3S_yyx, 3S_yyz, 3S_zzx, 3S_zzy, 6S_xyz),
+ *               plus C_x on xxx, yyx, zzx, C_y on yyy, xxy, zzy and C_z on zzz, xxz, yyz, where
+ *               S is the fully symmetrised second moment int (x-p)_j (x-p)_k n_i dA and
+ *               C = -4.5 t, t_k = sum_i S_iik
+ *   With d = p - q the far field is |d|^-3 (d.N + |d|^-2 (q1 + |d|^-2 q2)), q1 the quadratic form
+ *   of A and q2 the cubic of B: the expansion's constant terms tr M1 and C.d are folded into
+ *   them through |d|^2 / |d|^2.
+ *   23    link  internal slot: (child node << 4) | occupied slots of the child node (bit 31 clear);
+ *               leaf slot: FWN_LINK_LEAF | (first << 3) | (count - 1), the slot's triangles being
+ *               positions [first, first + count) of the sorted triangle array (count <= 8);
+ *               unoccupied slot: FWN_LINK_EMPTY (all other words 0)
+ * Triangles are stored in tree order, FWN_TRI_WORDS words each: v0, v1, v2 (9 floats), original
+ * facet index (int32), then two floats of the FP64 near-field rule of the leaf whose first
+ * triangle this is (0 for the other triangles of a bucket): (sqrt(r2) + tau)^2 and
+ * tau^2 |N|^2, with tau the mesh's FP64 distance (fwn_options.near_fp64_rel).
+ * Limits: fewer than 2^28 - 1 triangles and 2^27 nodes per engine.
  */
 #define FWN_WIDTH 8 /* maximum tree_width */
-#define FWN_SLOT_WORDS 32
+#define FWN_SLOT_WORDS 24
 #define FWN_TRI_WORDS 12
+#define FWN_LINK_LEAF 0x80000000u
+#define FWN_LINK_EMPTY 0xFFFFFFFFu
 
 enum {
-    FWN_W_PX = 0, FWN_W_PY = 1, FWN_W_PZ = 2, FWN_W_R2 = 3,
-    FWN_W_NX = 4, FWN_W_NY = 5, FWN_W_NZ = 6, FWN_W_TR = 7,
-    FWN_W_A = 8,  /* 6 words */
-    FWN_W_B = 14, /* 10 words */
-    FWN_W_C = 24, /* 3 words */
-    FWN_W_CHILD = 27, FWN_W_FIRST = 28, FWN_W_COUNT = 29, FWN_W_CHILD_NS = 30,
-    FWN_W_THR = 31 /* beta^2 r^2 in float: the far-field threshold for the engine's accuracy scale */
+    FWN_W_PX = 0, FWN_W_PY = 1, FWN_W_PZ = 2,
+    FWN_W_THR = 3, /* beta^2 r^2 in float: the far-field threshold for the engine's accuracy scale */
+    FWN_W_NX = 4, FWN_W_NY = 5, FWN_W_NZ = 6,
+    FWN_W_A = 7,  /* 6 words */
+    FWN_W_B = 13, /* 10 words */
+    FWN_W_LINK = 23
 };
+
+/* words of a stored triangle */
+enum { FWN_T_FACET = 9, FWN_T_NEAR2 = 10, FWN_T_PLANE = 11 };
 
 typedef struct fwn_tree_info {
     int64_t num_meshes;

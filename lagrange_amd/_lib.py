@@ -30,7 +30,7 @@ FWN_MOMENTS_LEVELS = 0
 FWN_MOMENTS_CLIMB = 1
 
 MAX_WIDTH = 8
-SLOT_WORDS = 32
+SLOT_WORDS = 24
 TRI_WORDS = 12
 
 _STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "NO_DEVICE", 4: "HIP_ERROR",

@@ -100,6 +100,7 @@ def build_io(verbose: bool = False) -> str:
 
 CXX_TEST = os.path.join(CXX_DIR, "build", "test_fast_winding_number")
 CXX_EXAMPLES = ("fix_orientation", "sample_points_in_mesh")
+CXX_BENCH = os.path.join(CXX_DIR, "build", "bench_point_calls")
 
 
 def build_cxx(verbose: bool = False) -> str:
@@ -130,6 +131,11 @@ def build_cxx(verbose: bool = False) -> str:
                      os.path.join(CXX_DIR, "examples", name + ".cpp")] + inc +
                     ["-L", HERE, "-llagrange_winding", "-lfwn", "-lfwn_io", "-Wl,-rpath," + HERE,
                      "-Wl,-rpath-link,/opt/rocm/lib"])
+    # the reference's per-point benchmark loop over the drop-in class
+    cmds.append(["g++", "-std=c++17", "-O2", "-Wall", "-pthread", "-o", CXX_BENCH,
+                 os.path.join(CXX_DIR, "bench", "bench_point_calls.cpp")] + inc +
+                ["-L", HERE, "-llagrange_winding", "-lfwn", "-lfwn_io", "-Wl,-rpath," + HERE,
+                 "-Wl,-rpath-link,/opt/rocm/lib"])
     for cmd in cmds:
         if verbose:
             print(" ".join(cmd), flush=True)

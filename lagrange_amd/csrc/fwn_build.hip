@@ -388,9 +388,9 @@ __global__ __launch_bounds__(BT) void k_leaf_records(const uint32_t* __restrict_
     uint32_t* tw = tris + j * kTriWords;
     for (int k = 0; k < 3; ++k)
         for (int d = 0; d < 3; ++d) tw[3 * k + d] = __float_as_uint(v[k][d]);
-    tw[9] = (uint32_t)(t - f_off[m]);
-    tw[10] = 0u;
-    tw[11] = 0u;
+    tw[FWN_T_FACET] = (uint32_t)(t - f_off[m]);
+    tw[FWN_T_NEAR2] = 0u; // the leaf data of a leaf's first triangle is written with the slots
+    tw[FWN_T_PLANE] = 0u;
     tri_record<float>(v, rec + (n - 1 + j) * kRecWords);
 }
 
@@ -610,7 +610,9 @@ __global__ __launch_bounds__(BT) void k_write_slots(int64_t W, int width, const 
                                                     const double* __restrict__ rec,
                                                     const int32_t* __restrict__ first,
                                                     const int32_t* __restrict__ last, int64_t n, float beta2,
-                                                    uint32_t* __restrict__ slots, float* __restrict__ boxes)
+                                                    const int64_t* __restrict__ f_off, int M,
+                                                    const float* __restrict__ minfo, uint32_t* __restrict__ slots,
+                                                    float* __restrict__ boxes, uint32_t* __restrict__ tris)
 {
     int64_t idx = (int64_t)blockIdx.x * BT + threadIdx.x;
     if (idx >= W * width) return;
@@ -621,7 +623,8 @@ __global__ __launch_bounds__(BT) void k_write_slots(int64_t W, int width, const 
     int64_t src = g * kWidth + s; // collapse scratch keeps kWidth entries per node
     int32_t u = wide_slot_bin[src];
     if (s >= wide_ns[g] || u < 0) {
-        for (int k = 0; k < kSlotWords; ++k) w[k] = 0u;
+        for (int k = 0; k < FWN_W_LINK; ++k) w[k] = 0u;
+        w[FWN_W_LINK] = FWN_LINK_EMPTY;
         for (int k = 0; k < 6; ++k) bx[k] = 0.0f;
         return;
     }
@@ -631,32 +634,33 @@ __global__ __launch_bounds__(BT) void k_write_slots(int64_t W, int width, const 
         bx[d] = __double2float_rd(r[R_LO + d]);
         bx[3 + d] = __double2float_ru(r[R_HI + d]);
     }
-    float f[27];
     double r2 = 0.0;
     for (int d = 0; d < 3; ++d) {
         double a = fabs(r[R_P + d] - r[R_LO + d]), b = fabs(r[R_HI + d] - r[R_P + d]);
         double e = a > b ? a : b;
         r2 += e * e;
     }
-    f[0] = (float)r[R_P];
-    f[1] = (float)r[R_P + 1];
-    f[2] = (float)r[R_P + 2];
-    f[3] = nextafterf((float)r2, INFINITY);
-    f[4] = (float)r[R_N];
-    f[5] = (float)r[R_N + 1];
-    f[6] = (float)r[R_N + 2];
+    const float r2f = nextafterf((float)r2, INFINITY); // rounded up: still a bound
+    float f[FWN_W_LINK];
+    f[FWN_W_PX] = (float)r[R_P];
+    f[FWN_W_PY] = (float)r[R_P + 1];
+    f[FWN_W_PZ] = (float)r[R_P + 2];
+    f[FWN_W_THR] = beta2 * r2f; // the query's far-field threshold
+    f[FWN_W_NX] = (float)r[R_N];
+    f[FWN_W_NY] = (float)r[R_N + 1];
+    f[FWN_W_NZ] = (float)r[R_N + 2];
     // tr and C are folded into the homogeneous quadratic and cubic: tr = tr |d|^2 / |d|^2 adds
     // tr to the diagonal of A, (C.d) = (C.d) |d|^2 / |d|^2 adds C_i to the cubic monomials
-    // d_i d_j d_j; the far field then needs no constant terms (words 7 and 24-26 keep tr and C)
-    const double* M = r + R_M1;
-    const double tr = M[0] + M[4] + M[8];
-    f[7] = (float)tr;
-    f[8] = (float)(-3.0 * M[0] + tr);
-    f[9] = (float)(-3.0 * M[4] + tr);
-    f[10] = (float)(-3.0 * M[8] + tr);
-    f[11] = (float)(-3.0 * (M[1] + M[3]));
-    f[12] = (float)(-3.0 * (M[5] + M[7]));
-    f[13] = (float)(-3.0 * (M[6] + M[2]));
+    // d_i d_j d_j; the far field then needs no constant terms
+    const double* M1 = r + R_M1;
+    const double tr = M1[0] + M1[4] + M1[8];
+    float* A = f + FWN_W_A;
+    A[0] = (float)(-3.0 * M1[0] + tr);
+    A[1] = (float)(-3.0 * M1[4] + tr);
+    A[2] = (float)(-3.0 * M1[8] + tr);
+    A[3] = (float)(-3.0 * (M1[1] + M1[3]));
+    A[4] = (float)(-3.0 * (M1[5] + M1[7]));
+    A[5] = (float)(-3.0 * (M1[6] + M1[2]));
     const double* S = r + R_S;
     const double sc[10] = {7.5, 7.5, 7.5, 22.5, 22.5, 22.5, 22.5, 22.5, 22.5, 45.0};
     const double cx = -4.5 * (S[0] + S[5] + S[7]);
@@ -664,20 +668,28 @@ __global__ __launch_bounds__(BT) void k_write_slots(int64_t W, int width, const 
     const double cz = -4.5 * (S[4] + S[6] + S[2]);
     // monomials: xxx yyy zzz xxy xxz yyx yyz zzx zzy xyz
     const double fold[10] = {cx, cy, cz, cy, cz, cx, cz, cx, cy, 0.0};
-    for (int k = 0; k < 10; ++k) f[14 + k] = (float)(sc[k] * S[k] + fold[k]);
-    f[24] = (float)cx;
-    f[25] = (float)cy;
-    f[26] = (float)cz;
-    for (int k = 0; k < 27; ++k) w[k] = __float_as_uint(f[k]);
+    for (int k = 0; k < 10; ++k) f[FWN_W_B + k] = (float)(sc[k] * S[k] + fold[k]);
+    for (int k = 0; k < FWN_W_LINK; ++k) w[k] = __float_as_uint(f[k]);
     // a leaf slot is any slot without a wide child: a single triangle or a bucket of at most
     // leaf_max triangles (an internal binary node); both cover a contiguous sorted range
-    bool bin_leaf = u >= n - 1;
-    int32_t cw = child_wide[src];
-    w[FWN_W_CHILD] = (uint32_t)cw;
-    w[FWN_W_FIRST] = (uint32_t)(bin_leaf ? (int32_t)(u - (n - 1)) : first[u]);
-    w[FWN_W_COUNT] = (uint32_t)(bin_leaf ? 1 : last[u] - first[u] + 1);
-    w[FWN_W_CHILD_NS] = (uint32_t)(cw >= 0 ? wide_ns[cw] : 0);
-    w[FWN_W_THR] = __float_as_uint(beta2 * f[3]); // the query's far-field threshold
+    const int32_t cw = child_wide[src];
+    if (cw >= 0) {
+        w[FWN_W_LINK] = ((uint32_t)cw << 4) | (uint32_t)wide_ns[cw];
+        return;
+    }
+    const bool bin_leaf = u >= n - 1;
+    const int64_t tf = bin_leaf ? (u - (n - 1)) : first[u];
+    const int32_t tc = bin_leaf ? 1 : last[u] - first[u] + 1;
+    w[FWN_W_LINK] = FWN_LINK_LEAF | ((uint32_t)tf << 3) | (uint32_t)(tc - 1);
+    // the leaf's FP64 near-field rule, in its first triangle: q is within tau of the leaf's
+    // bounding sphere when |d|^2 < (sqrt(r2) + tau)^2 and, for a single triangle, within tau of
+    // its plane when (d.N)^2 < tau^2 |N|^2
+    const float tau = minfo[8 * mesh_of(f_off, M, tf) + 6];
+    const float rr = sqrtf(r2f) + tau;
+    const float nn = fmaf(f[FWN_W_NX], f[FWN_W_NX], fmaf(f[FWN_W_NY], f[FWN_W_NY], f[FWN_W_NZ] * f[FWN_W_NZ]));
+    uint32_t* tw = tris + tf * kTriWords;
+    tw[FWN_T_NEAR2] = __float_as_uint(rr * rr);
+    tw[FWN_T_PLANE] = __float_as_uint((tau * tau) * nn);
 }
 
 } // namespace
@@ -696,7 +708,8 @@ void build_engine(Engine& e, int32_t M, const float* V_host, const int64_t* v_of
     hipStream_t s = e.stream;
     const int64_t nV = v_off[M];
     const int64_t nF = f_off[M];
-    if (nF > (int64_t)INT32_MAX / 2) fail(FWN_LIMIT_EXCEEDED, "more than 2^30 triangles");
+    // a leaf slot's link holds its first triangle in 28 bits
+    if (nF >= ((int64_t)1 << 28) - 1) fail(FWN_LIMIT_EXCEEDED, "2^28 - 1 or more triangles");
     e.n_meshes = M;
     e.n_tris = nF;
     e.h_f_off.assign(f_off, f_off + M + 1);
@@ -928,9 +941,11 @@ void build_engine(Engine& e, int32_t M, const float* V_host, const int64_t* v_of
     e.max_depth = depth;
     e.slots.alloc(sizeof(uint32_t) * width * kSlotWords * (size_t)std::max<int64_t>(W, 1));
     e.boxes.alloc(sizeof(float) * width * 6 * (size_t)std::max<int64_t>(W, 1));
+    if (W >= ((int64_t)1 << 27)) fail(FWN_LIMIT_EXCEEDED, "more than 2^27 tree nodes");
     hipLaunchKernelGGL(k_write_slots, dim3(grid_blocks(W * width, BT)), dim3(BT), 0, s, W, width, wsb.as<int32_t>(),
                        wns.as<int32_t>(), cw.as<int32_t>(), rec.as<double>(), first.as<int32_t>(), last.as<int32_t>(),
-                       n, e.opt.accuracy_scale * e.opt.accuracy_scale, e.slots.as<uint32_t>(), e.boxes.as<float>());
+                       n, e.opt.accuracy_scale * e.opt.accuracy_scale, dfo.as<int64_t>(), M,
+                       e.mesh_info.as<float>(), e.slots.as<uint32_t>(), e.boxes.as<float>(), e.tris.as<uint32_t>());
     FWN_LAUNCH_CHECK("k_write_slots");
     std::vector<int32_t> ns_root(level0.size());
     if (!level0.empty())

@@ -281,24 +281,24 @@ __global__ __launch_bounds__(64) void k_closest(const uint32_t* __restrict__ slo
             const uint32_t* nb = slots + (size_t)node * (size_t)width * kSlotWords;
             const float* nbx = boxes + (size_t)node * (size_t)width * 6;
             double db[kWidth];
-            int32_t child[kWidth];
+            uint32_t link[kWidth]; // leaf slots have the top bit set
 #pragma unroll
             for (int s = 0; s < kWidth; ++s) {
                 db[s] = INFINITY;
-                child[s] = 0;
+                link[s] = 0u;
                 if (s < ns) {
                     db[s] = box_dist2(nbx + 6 * s, qx, qy, qz);
-                    child[s] = (int32_t)nb[s * kSlotWords + FWN_W_CHILD];
+                    link[s] = nb[s * kSlotWords + FWN_W_LINK];
                 }
             }
             // leaf slots first
 #pragma unroll
             for (int s = 0; s < kWidth; ++s) {
-                if (s < ns && child[s] < 0) {
+                if (s < ns && (link[s] & FWN_LINK_LEAF) != 0u) {
                     const bool go = act && db[s] < best;
                     if (__ballot(go) != 0ull) {
-                        const int32_t first = (int32_t)nb[s * kSlotWords + FWN_W_FIRST];
-                        const int32_t cnt = (int32_t)nb[s * kSlotWords + FWN_W_COUNT];
+                        const int32_t first = (int32_t)((link[s] & ~FWN_LINK_LEAF) >> 3);
+                        const int32_t cnt = (int32_t)(link[s] & 7u) + 1;
                         for (int32_t t = first; t < first + cnt; ++t) {
                             const float* tw = reinterpret_cast<const float*>(tris + (size_t)t * kTriWords);
                             // a triangle whose box is farther than the lane's best cannot improve it;
@@ -324,7 +324,7 @@ __global__ __launch_bounds__(64) void k_closest(const uint32_t* __restrict__ slo
             float key[kWidth];
 #pragma unroll
             for (int s = 0; s < kWidth; ++s) {
-                const bool go = s < ns && child[s] >= 0 && act && db[s] < best;
+                const bool go = s < ns && (link[s] & FWN_LINK_LEAF) == 0u && act && db[s] < best;
                 pm[s] = __ballot(go);
                 // ordering key: the box distance seen by the slot's first participating lane (the
                 // wave's points are close together; the order only steers the walk)
@@ -346,8 +346,7 @@ __global__ __launch_bounds__(64) void k_closest(const uint32_t* __restrict__ slo
                 for (int s = 0; s < kWidth; ++s) {
                     if (s == sel) {
                         if (sp < WaveStack<NB>::kCap) {
-                            const uint32_t cns = nb[s * kSlotWords + FWN_W_CHILD_NS];
-                            S.push(sp, lane, ((uint32_t)child[s] << 4) | cns, pm[s]);
+                            S.push(sp, lane, link[s], pm[s]); // (child << 4) | its slot count
                             ++sp;
                         } else {
                             overflow = true;
@@ -364,7 +363,7 @@ __global__ __launch_bounds__(64) void k_closest(const uint32_t* __restrict__ slo
         int32_t fid = -1;
         double cp[3] = {NAN, NAN, NAN};
         if (btri >= 0) {
-            fid = (int32_t)tris[(size_t)btri * kTriWords + 9];
+            fid = (int32_t)tris[(size_t)btri * kTriWords + FWN_T_FACET];
             closest_on(reinterpret_cast<const float*>(tris + (size_t)btri * kTriWords), bs, bt, cp);
         }
         tri_out[orig] = fid;

@@ -140,6 +140,12 @@ void radix_sort_pairs(uint64_t*& keys, uint32_t*& vals, uint64_t*& keys_alt, uin
 // then the permutation), so the key kernel need not write them.
 void radix_sort_pairs32(uint32_t*& keys, uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt, int64_t n,
                         int begin_bit, int end_bit, bool vals_are_index, DBuf& hist, DBuf& rowsum, hipStream_t s);
+// The same over caller-provided histogram scratch (no allocation): hist holds
+// radix_sort_hist_words(n) int32, rowsum 256 int32.
+int64_t radix_sort_hist_words(int64_t n);
+void radix_sort_pairs32_scratch(uint32_t*& keys, uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt, int64_t n,
+                                int begin_bit, int end_bit, bool vals_are_index, int32_t* hist, int32_t* rowsum,
+                                hipStream_t s);
 // exclusive scan of int32 (in may alias out); returns nothing, total in out[n] if out has n+1 slots
 void exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, DBuf& tmp, hipStream_t s);
 
@@ -160,6 +166,13 @@ void build_topology_sah(const float* cen, const float* tbox, int64_t nF, const i
 void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t nQ, float* omega, uint8_t* inside,
                  int32_t* stats, bool device_ptrs);
 void run_bruteforce(Engine& e, const float* Q, int64_t nQ, double* out, bool device_ptrs);
+// Pieces of a query that uses no engine scratch (single-mesh engine, device points): the 32-bit
+// point keys, and the traversal in sorted order `perm` with FP64 near leaves evaluated in place
+// (no work list); meta holds q_off {0, nQ}, b_off {0, batches} and the mesh order {0} (5 int64
+// words), flag one int32 (set on stack overflow).  Results equal run_queries' bit for bit.
+void launch_point_keys(Engine& e, const float* dQ, int64_t nQ, uint32_t* keys, hipStream_t s);
+void launch_traverse_inplace(Engine& e, int banks, const float* dQ, const uint32_t* perm, const int64_t* meta,
+                             int64_t nQ, float* omega, uint8_t* inside, int32_t* flag, hipStream_t s);
 // Morton order of nQ device points of a single-mesh engine: returns the permutation (sorted
 // position -> original index) in the engine's query scratch, valid until the next query call.
 const uint32_t* sort_points_single(Engine& e, const float* dQ, int64_t nQ);
@@ -265,12 +278,22 @@ __host__ __device__ inline float ord2f(int32_t i)
     return f;
 }
 
+// v_writelane_b32: returns `old` with lane `lane` replaced by the wave-uniform `v` (the LLVM
+// intrinsic, which clang exposes under no builtin name)
+extern "C" __device__ int fwn_llvm_writelane(int v, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+__device__ __forceinline__ uint32_t wave_writelane(uint32_t v, int lane, uint32_t old)
+{
+    return (uint32_t)fwn_llvm_writelane((int)v, lane, (int)old);
+}
+
 // Wave-private traversal stack of (node entry, 64-bit lane mask), one entry per lane and bank:
 // entry k lives in lane k % 64 of bank k / 64.  NB = 1 (64 entries, enough for trees with
-// 1 + (width - 1) * depth <= 64) or 3 (kStack entries).  A push is a lane select, a pop is
-// v_readlane; every bank is updated / read under uniform predicates with no branches, so nothing
-// invites the optimiser to turn the banks into an indexed private array in scratch memory.  A
-// push past the capacity writes no lane (callers report it as an overflow).
+// 1 + (width - 1) * depth <= 64) or 3 (kStack entries).  The entry and the mask are
+// wave-uniform, so a push is one v_writelane per word (the written register is tied to the old
+// one: no copies at the join of a conditional push) and a pop is one v_readlane per word; the
+// bank is chosen by uniform branches, so nothing invites the optimiser to turn the banks into an
+// indexed private array in scratch memory.  A push past the capacity writes no lane (callers
+// report it as an overflow).
 template <int NB>
 struct WaveStack;
 
@@ -278,12 +301,12 @@ template <>
 struct WaveStack<1> {
     static constexpr int kCap = 64;
     uint32_t n0 = 0, l0 = 0, h0 = 0;
-    __device__ __forceinline__ void push(int sp, int lane, uint32_t e, uint64_t mk)
+    __device__ __forceinline__ void push(int sp, int /*lane*/, uint32_t e, uint64_t mk)
     {
-        const bool me = lane == sp;
-        n0 = me ? e : n0;
-        l0 = me ? (uint32_t)mk : l0;
-        h0 = me ? (uint32_t)(mk >> 32) : h0;
+        if (sp >= kCap) return;
+        n0 = wave_writelane(e, sp, n0);
+        l0 = wave_writelane((uint32_t)mk, sp, l0);
+        h0 = wave_writelane((uint32_t)(mk >> 32), sp, h0);
     }
     __device__ __forceinline__ void pop(int sp, uint32_t& e, uint64_t& mk) const
     {
@@ -298,20 +321,23 @@ template <>
 struct WaveStack<3> {
     static constexpr int kCap = kStack;
     uint32_t n0 = 0, n1 = 0, n2 = 0, l0 = 0, l1 = 0, l2 = 0, h0 = 0, h1 = 0, h2 = 0;
-    __device__ __forceinline__ void push(int sp, int lane, uint32_t e, uint64_t mk)
+    __device__ __forceinline__ void push(int sp, int /*lane*/, uint32_t e, uint64_t mk)
     {
-        const bool me = lane == (sp & 63);
-        const bool b0 = me && sp < 64, b1 = me && sp >= 64 && sp < 128, b2 = me && sp >= 128 && sp < kStack;
+        const int l = sp & 63;
         const uint32_t mlo = (uint32_t)mk, mhi = (uint32_t)(mk >> 32);
-        n0 = b0 ? e : n0;
-        l0 = b0 ? mlo : l0;
-        h0 = b0 ? mhi : h0;
-        n1 = b1 ? e : n1;
-        l1 = b1 ? mlo : l1;
-        h1 = b1 ? mhi : h1;
-        n2 = b2 ? e : n2;
-        l2 = b2 ? mlo : l2;
-        h2 = b2 ? mhi : h2;
+        if (sp < 64) {
+            n0 = wave_writelane(e, l, n0);
+            l0 = wave_writelane(mlo, l, l0);
+            h0 = wave_writelane(mhi, l, h0);
+        } else if (sp < 128) {
+            n1 = wave_writelane(e, l, n1);
+            l1 = wave_writelane(mlo, l, l1);
+            h1 = wave_writelane(mhi, l, h1);
+        } else if (sp < kStack) {
+            n2 = wave_writelane(e, l, n2);
+            l2 = wave_writelane(mlo, l, l2);
+            h2 = wave_writelane(mhi, l, h2);
+        }
     }
     __device__ __forceinline__ void pop(int sp, uint32_t& e, uint64_t& mk) const
     {

@@ -109,50 +109,67 @@ __device__ __forceinline__ double tri_f64(const float* __restrict__ t, float qx,
     return 2.0 * atan2(num, den);
 }
 
-// One slot's 32 words held in SGPRs (two 16-dword scalar loads issued together at the top of
-// the slot).  The other waves of the SIMD (8 resident) hide the load latency; a prefetch of the
-// next slot into a second register set costs 16 scalar moves per slot and 24 more SGPRs.
+// One slot's 24 words held in SGPRs (a 16-dword and an 8-dword scalar load issued together at
+// the top of the slot).  The other waves of the SIMD (8 resident) hide the load latency.
 typedef uint32_t v16u __attribute__((ext_vector_type(16)));
+typedef uint32_t v8u __attribute__((ext_vector_type(8)));
+typedef v16u v16u_a4 __attribute__((aligned(4))); // slots are 96-byte records: 4-byte alignment
+typedef v8u v8u_a4 __attribute__((aligned(4)));
 struct SlotRegs {
-    v16u a, b;
+    v16u a;
+    v8u b;
 };
 
 __device__ __forceinline__ SlotRegs load_slot(const uint32_t* __restrict__ p)
 {
-    const v16u* q = reinterpret_cast<const v16u*>(p);
-    return SlotRegs{q[0], q[1]};
+    return SlotRegs{*reinterpret_cast<const v16u_a4*>(p), *reinterpret_cast<const v8u_a4*>(p + 16)};
 }
 
-__device__ __forceinline__ float sw(const SlotRegs& r, int k) { return __uint_as_float(k < 16 ? r.a[k] : r.b[k - 16]); }
-__device__ __forceinline__ int32_t si(const SlotRegs& r, int k) { return (int32_t)(k < 16 ? r.a[k] : r.b[k - 16]); }
+__device__ __forceinline__ uint32_t su(const SlotRegs& r, int k) { return k < 16 ? r.a[k] : r.b[k - 16]; }
+__device__ __forceinline__ float sw(const SlotRegs& r, int k) { return __uint_as_float(su(r, k)); }
 
-// order-2 far field of one slot held in registers, added to om:
+// order-2 far field of one slot, split into t and inv3 (the caller adds fmaf(t, inv3, omega)):
 //   omega += inv3 (o0 + inv2 (q1 + inv2 q2)),  inv = 1 / |d|,
 // o0 = d.N, q1 = d^T A d, q2 = B(d, d, d).  The build folds the constant terms (tr M1 of o0 and
 // C.d of q1) into A and B through |d|^2 / |d|^2, so every product has one coefficient operand
 // and no coefficient needs a move into a vector register; q1 and q2 are evaluated in Horner form
-// over the packed coefficients (no monomials needed)
-__device__ __forceinline__ float far_field_acc(const SlotRegs& r, float dx, float dy, float dz, float d2, float om)
+// over the packed coefficients (no monomials needed).  W(k) reads slot word k: scalar registers
+// in the batched traversal, per-lane registers in the single-point kernel (the same operations,
+// so the same bits).
+template <class W>
+__device__ __forceinline__ void far_field_terms(const W& w, float dx, float dy, float dz, float d2, float& t,
+                                                float& inv3)
 {
     const float inv = __builtin_amdgcn_rsqf(d2);
     const float inv2 = inv * inv;
-    const float inv3 = inv2 * inv;
-    const float o0 = fmaf(dx, sw(r, FWN_W_NX), fmaf(dy, sw(r, FWN_W_NY), dz * sw(r, FWN_W_NZ)));
+    inv3 = inv2 * inv;
+    const float o0 = fmaf(dx, w(FWN_W_NX), fmaf(dy, w(FWN_W_NY), dz * w(FWN_W_NZ)));
     const float zz = dz * dz; // shared with the slot test's |d|^2
     // A: xx yy zz xy yz zx;  q1 = A_zz zz + x (A_xx x + A_xy y + A_zx z) + y (A_yy y + A_yz z)
-    const float a1 = fmaf(sw(r, FWN_W_A), dx, fmaf(sw(r, FWN_W_A + 3), dy, sw(r, FWN_W_A + 5) * dz));
-    const float b1 = fmaf(sw(r, FWN_W_A + 1), dy, sw(r, FWN_W_A + 4) * dz);
-    const float q1 = fmaf(dy, b1, fmaf(dx, a1, sw(r, FWN_W_A + 2) * zz));
+    const float a1 = fmaf(w(FWN_W_A), dx, fmaf(w(FWN_W_A + 3), dy, w(FWN_W_A + 5) * dz));
+    const float b1 = fmaf(w(FWN_W_A + 1), dy, w(FWN_W_A + 4) * dz);
+    const float q1 = fmaf(dy, b1, fmaf(dx, a1, w(FWN_W_A + 2) * zz));
     // B: xxx yyy zzz xxy xxz yyx yyz zzx zzy xyz;
     // q2 = zz (B_zzx x + B_zzy y + B_zzz z) + x (x (B_xxx x + B_xxy y + B_xxz z) + y (B_yyx y + B_xyz z))
     //      + y y (B_yyy y + B_yyz z)
-    const float L = fmaf(sw(r, FWN_W_B + 7), dx, fmaf(sw(r, FWN_W_B + 8), dy, sw(r, FWN_W_B + 2) * dz));
-    const float P = fmaf(sw(r, FWN_W_B), dx, fmaf(sw(r, FWN_W_B + 3), dy, sw(r, FWN_W_B + 4) * dz));
-    const float Qv = fmaf(sw(r, FWN_W_B + 5), dy, sw(r, FWN_W_B + 9) * dz);
+    const float L = fmaf(w(FWN_W_B + 7), dx, fmaf(w(FWN_W_B + 8), dy, w(FWN_W_B + 2) * dz));
+    const float P = fmaf(w(FWN_W_B), dx, fmaf(w(FWN_W_B + 3), dy, w(FWN_W_B + 4) * dz));
+    const float Qv = fmaf(w(FWN_W_B + 5), dy, w(FWN_W_B + 9) * dz);
     const float X = fmaf(dy, Qv, dx * P);
-    const float R = fmaf(sw(r, FWN_W_B + 1), dy, sw(r, FWN_W_B + 6) * dz);
+    const float R = fmaf(w(FWN_W_B + 1), dy, w(FWN_W_B + 6) * dz);
     const float q2 = fmaf(dy * dy, R, fmaf(dx, X, zz * L));
-    const float t = fmaf(inv2, fmaf(inv2, q2, q1), o0);
+    t = fmaf(inv2, fmaf(inv2, q2, q1), o0);
+}
+
+struct SlotWords {
+    const SlotRegs& r;
+    __device__ __forceinline__ float operator()(int k) const { return sw(r, k); }
+};
+
+__device__ __forceinline__ float far_field_acc(const SlotRegs& r, float dx, float dy, float dz, float d2, float om)
+{
+    float t, inv3;
+    far_field_terms(SlotWords{r}, dx, dy, dz, d2, t, inv3);
     return fmaf(t, inv3, om);
 }
 
@@ -180,8 +197,11 @@ __device__ __forceinline__ void pad_chunk(const ItemChunk& ch, int lane, uint32_
 }
 
 // One batch of up to bpts <= 64 points (one per lane) of one mesh, walked through that mesh's
-// tree (bpts is 64 except in small calls, which spread their points over more waves).
-template <bool STATS, int NB>
+// tree (bpts is 64 except in small calls, which spread their points over more waves).  INL:
+// FP64 near leaves are evaluated in place (each leaf's FP64 sum in 2^-40 fixed point, the
+// point's total added once at the end: the value the deferred pass gives) instead of being
+// deferred to the work list; for calls that use no work list.
+template <bool STATS, int NB, bool INL = false>
 __device__ __forceinline__ void traverse_batch(
     int64_t batch, const uint32_t* __restrict__ slots, int width, const uint32_t* __restrict__ tris,
     const int32_t* __restrict__ roots, const float* __restrict__ minfo, const float* __restrict__ Q,
@@ -206,10 +226,10 @@ __device__ __forceinline__ void traverse_batch(
     const int32_t root = roots[2 * m];
     const int32_t root_ns = roots[2 * m + 1];
     const float tau = minfo[8 * m + 6];
-    const float tau2 = tau * tau;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
     float om = 0.0f;
+    unsigned long long acc64 = 0ull; // INL: FP64 near leaves in 2^-40 fixed point
     int32_t n_test = 0, n_far = 0, n_t32 = 0, n_t64 = 0;
     if (root >= 0) {
         WaveStack<NB> S;
@@ -229,7 +249,6 @@ __device__ __forceinline__ void traverse_batch(
             auto slot = [&](const SlotRegs& cur) __attribute__((always_inline)) {
                 const float dx = sw(cur, FWN_W_PX) - qx, dy = sw(cur, FWN_W_PY) - qy, dz = sw(cur, FWN_W_PZ) - qz;
                 const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                const float r2 = sw(cur, FWN_W_R2);
                 const bool far = d2 > sw(cur, FWN_W_THR); // beta^2 r^2, written by the build
                 // near lanes: active and not far (scalar mask arithmetic on the compare's ballot,
                 // taken before the divergent far-field branch)
@@ -239,66 +258,75 @@ __device__ __forceinline__ void traverse_batch(
                     om = far_field_acc(cur, dx, dy, dz, d2, om);
                     if (STATS) ++n_far;
                 }
-                if (nearm != 0ull) {
-                    const int32_t child = si(cur, FWN_W_CHILD);
-                    if (child < 0) {
-                        const int32_t first = si(cur, FWN_W_FIRST);
-                        const int32_t tcnt = si(cur, FWN_W_COUNT);
-                        const bool nl = __builtin_amdgcn_inverse_ballot_w64(nearm);
-                        // FP64 for points within tau of the leaf's bounding sphere and, for a single
-                        // triangle, within tau of its plane (p lies on that plane)
-                        const float rr = sqrtf(r2) + tau;
-                        const float nx = sw(cur, FWN_W_NX), ny = sw(cur, FWN_W_NY), nz = sw(cur, FWN_W_NZ);
-                        const float nd = fmaf(dx, nx, fmaf(dy, ny, dz * nz));
-                        const float nn = fmaf(nx, nx, fmaf(ny, ny, nz * nz));
-                        const bool use64 = nl && tau > 0.0f && d2 < rr * rr && (tcnt > 1 || nd * nd < tau2 * nn);
-                        const bool use32 = nl && !use64;
-                        if (__ballot(use32) != 0ull) {
-                            for (int32_t t = first; t < first + tcnt; ++t) {
-                                const float* tw = reinterpret_cast<const float*>(tris + (size_t)t * kTriWords);
-                                if (use32) {
-                                    om += tri_f32(tw, qx, qy, qz);
-                                    if (STATS) ++n_t32;
-                                }
+                const uint32_t link = su(cur, FWN_W_LINK);
+                const bool leaf = (link & FWN_LINK_LEAF) != 0u;
+                if (nearm != 0ull && leaf) {
+                    const int32_t first = (int32_t)((link & ~FWN_LINK_LEAF) >> 3);
+                    const int32_t tcnt = (int32_t)(link & 7u) + 1;
+                    const bool nl = __builtin_amdgcn_inverse_ballot_w64(nearm);
+                    // FP64 for points within tau of the leaf's bounding sphere and, for a single
+                    // triangle, within tau of its plane (p lies on that plane); both thresholds are
+                    // stored in the leaf's first triangle
+                    const float* t0 = reinterpret_cast<const float*>(tris + (size_t)first * kTriWords);
+                    const float near2 = t0[FWN_T_NEAR2], plane = t0[FWN_T_PLANE];
+                    const float nd = fmaf(dx, sw(cur, FWN_W_NX), fmaf(dy, sw(cur, FWN_W_NY), dz * sw(cur, FWN_W_NZ)));
+                    const bool use64 = nl && tau > 0.0f && d2 < near2 && (tcnt > 1 || nd * nd < plane);
+                    const bool use32 = nl && !use64;
+                    if (__ballot(use32) != 0ull) {
+                        for (int32_t t = first; t < first + tcnt; ++t) {
+                            const float* tw = reinterpret_cast<const float*>(tris + (size_t)t * kTriWords);
+                            if (use32) {
+                                om += tri_f32(tw, qx, qy, qz);
+                                if (STATS) ++n_t32;
                             }
                         }
-                        const uint64_t m64 = __ballot(use64);
-                        if (m64 != 0ull) {
-                            // deferred: (point, leaf) items, evaluated in FP64 by k_near64
-                            const uint32_t c = (uint32_t)__popcll(m64);
-                            if (ch.fill + c > kItemChunk) {
-                                pad_chunk(ch, lane, items, item_cap);
-                                uint32_t nc = 0u;
-                                if (lane == 0) nc = atomicAdd(n_chunks, 1u);
-                                nc = __builtin_amdgcn_readfirstlane(nc);
-                                ch.base = min(nc, 0xFFFFFFFFu / kItemChunk) * kItemChunk;
-                                ch.fill = 0u;
-                            }
-                            if (use64) {
-                                const uint32_t k = ch.base + ch.fill + (uint32_t)__popcll(m64 & lt_mask);
-                                if (k < item_cap) {
-                                    items[3 * (size_t)k] = jpos;
-                                    items[3 * (size_t)k + 1] = (uint32_t)first;
-                                    items[3 * (size_t)k + 2] = (uint32_t)tcnt;
-                                }
-                                if (STATS) n_t64 += tcnt;
-                            }
-                            ch.fill += c;
-                        }
-                    } else {
-                        // the host only launches trees whose stack bound 1 + (width - 1) depth fits;
-                        // past the capacity a push writes no lane and the overflow is reported
-                        overflow |= sp >= WaveStack<NB>::kCap;
-                        const uint32_t cns = (uint32_t)si(cur, FWN_W_CHILD_NS);
-                        S.push(sp, lane, ((uint32_t)child << 4) | cns, nearm);
-                        ++sp;
                     }
+                    const uint64_t m64 = __ballot(use64);
+                    if (INL && m64 != 0ull) {
+                        if (use64) {
+                            double s64 = 0.0;
+                            for (int32_t t = first; t < first + tcnt; ++t)
+                                s64 += tri_f64(reinterpret_cast<const float*>(tris + (size_t)t * kTriWords), qx, qy, qz);
+                            acc64 += (unsigned long long)__double2ll_rn(s64 * kFixScale);
+                            if (STATS) n_t64 += tcnt;
+                        }
+                    } else if (m64 != 0ull) {
+                        // deferred: (point, leaf) items, evaluated in FP64 by k_near64
+                        const uint32_t c = (uint32_t)__popcll(m64);
+                        if (ch.fill + c > kItemChunk) {
+                            pad_chunk(ch, lane, items, item_cap);
+                            uint32_t nc = 0u;
+                            if (lane == 0) nc = atomicAdd(n_chunks, 1u);
+                            nc = __builtin_amdgcn_readfirstlane(nc);
+                            ch.base = min(nc, 0xFFFFFFFFu / kItemChunk) * kItemChunk;
+                            ch.fill = 0u;
+                        }
+                        if (use64) {
+                            const uint32_t k = ch.base + ch.fill + (uint32_t)__popcll(m64 & lt_mask);
+                            if (k < item_cap) {
+                                items[3 * (size_t)k] = jpos;
+                                items[3 * (size_t)k + 1] = (uint32_t)first;
+                                items[3 * (size_t)k + 2] = (uint32_t)tcnt;
+                            }
+                            if (STATS) n_t64 += tcnt;
+                        }
+                        ch.fill += c;
+                    }
+                }
+                if (nearm != 0ull && !leaf) {
+                    // the host only launches trees whose stack bound 1 + (width - 1) depth fits;
+                    // past the capacity a push writes no lane and the overflow is reported.  The
+                    // link of an internal slot is the stack entry: (child << 4) | its slot count
+                    overflow |= sp >= WaveStack<NB>::kCap;
+                    S.push(sp, lane, link, nearm);
+                    ++sp;
                 }
             };
             for (int s = 0; s < ns; ++s) slot(load_slot(nb + s * kSlotWords));
         }
         if (overflow && lane == 0) atomicOr(flag, 1);
     }
+    if (INL && acc64 != 0ull) om = (float)((double)om + (double)(long long)acc64 * (1.0 / kFixScale));
     if (valid) {
         omega[orig] = om;
         if (inside) inside[orig] = ((double)om / (4.0 * 3.14159265358979323846) > 0.5) ? 1 : 0;
@@ -314,7 +342,7 @@ __device__ __forceinline__ void traverse_batch(
 // Grid launch: one wave per batch, XCD-aware block order.  Persistent launch: a grid sized to
 // the device's resident waves; each wave takes batches from a work counter until none is left
 // (the batch numbering puts the meshes with the most triangles first).
-template <bool STATS, int WAVES, bool PERSIST, int NB>
+template <bool STATS, int WAVES, bool PERSIST, int NB, bool INL = false>
 __global__ __launch_bounds__(WAVES * 64) void k_traverse(
     const uint32_t* __restrict__ slots, int width, const uint32_t* __restrict__ tris,
     const int32_t* __restrict__ roots, const float* __restrict__ minfo, const float* __restrict__ Q,
@@ -327,7 +355,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_traverse(
     if (!PERSIST) {
         const int64_t batch = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * WAVES + (threadIdx.x >> 6);
         if (batch >= n_batches) return; // whole wave
-        traverse_batch<STATS, NB>(batch, slots, width, tris, roots, minfo, Q, perm, q_off, b_off, mesh_order, M, bpts,
+        traverse_batch<STATS, NB, INL>(batch, slots, width, tris, roots, minfo, Q, perm, q_off, b_off, mesh_order, M, bpts,
                                   beta2, omega, inside, stats, items, n_chunks, item_cap, flag, ch);
     } else {
         for (;;) {
@@ -335,11 +363,11 @@ __global__ __launch_bounds__(WAVES * 64) void k_traverse(
             if ((threadIdx.x & 63) == 0) b = atomicAdd(work, 1u);
             b = __builtin_amdgcn_readfirstlane(b);
             if ((int64_t)b >= n_batches) break; // every wave leaves once the counter passes the end
-            traverse_batch<STATS, NB>((int64_t)b, slots, width, tris, roots, minfo, Q, perm, q_off, b_off, mesh_order, M,
+            traverse_batch<STATS, NB, INL>((int64_t)b, slots, width, tris, roots, minfo, Q, perm, q_off, b_off, mesh_order, M,
                                       bpts, beta2, omega, inside, stats, items, n_chunks, item_cap, flag, ch);
         }
     }
-    pad_chunk(ch, threadIdx.x & 63, items, item_cap); // close the wave's open chunk
+    if (!INL) pad_chunk(ch, threadIdx.x & 63, items, item_cap); // close the wave's open chunk
 }
 
 // FP64 solid angles of the deferred (point, leaf) items, summed per point in 2^-40 fixed
@@ -402,6 +430,132 @@ __global__ __launch_bounds__(256) void k_bruteforce(const uint32_t* __restrict__
     for (int64_t t = first; t < first + cnt; ++t)
         s += tri_f64(reinterpret_cast<const float*>(tris + t * kTriWords), qx, qy, qz);
     if (valid) out[i] = s;
+}
+
+// ---------------------------------------------------------------------------------------
+// Single-point calls (the reference's per-point solid_angle / is_inside): one wave per point,
+// the lanes of the wave taking the slots of a node in parallel.  A point walks the same tree in
+// the same order as in the batched traversal (children pushed in slot order, popped last-in-
+// first-out), so the walk is a chain of dependent node loads; here each node costs one load
+// round trip (lane s loads slot s) and one parallel test / far-field evaluation, instead of one
+// scalar load and one far-field evaluation per slot in turn.  The terms are then added in the
+// batched traversal's order (slot order, the same FP32 operations: far terms as fmaf(t, inv3,
+// omega), near FP32 triangles one by one) and the FP64 near leaves in 2^-40 fixed point, so a
+// point gets bit for bit the value a batched call gives it.  Points and the launch come in the
+// kernel arguments; results are written straight to pinned host memory.
+constexpr int kPointArgs = 16; // points per single-point launch
+
+struct PointArgs {
+    float q[kPointArgs][3];
+};
+
+struct LaneWords {
+    const float* w;
+    __device__ __forceinline__ float operator()(int k) const { return w[k]; }
+};
+
+__device__ __forceinline__ float readlane_f(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+template <int NB>
+__global__ __launch_bounds__(64) void k_point(const uint32_t* __restrict__ slots, int width,
+                                              const uint32_t* __restrict__ tris, int32_t root, int32_t root_ns,
+                                              float tau, PointArgs pa, float* __restrict__ omega,
+                                              uint8_t* __restrict__ inside, int32_t* __restrict__ flag)
+{
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const float qx = pa.q[i][0], qy = pa.q[i][1], qz = pa.q[i][2];
+    float om = 0.0f;
+    unsigned long long acc = 0ull; // FP64 near leaves, 2^-40 fixed point
+    if (root >= 0) {
+        WaveStack<NB> S;
+        S.push(0, lane, ((uint32_t)root << 4) | (uint32_t)root_ns, 0ull);
+        int sp = 1;
+        bool overflow = false;
+        while (sp > 0) {
+            --sp;
+            uint32_t e;
+            uint64_t unused;
+            S.pop(sp, e, unused);
+            const int ns = (int)(e & 15u);
+            if (ns == 0) continue;
+            const uint32_t* nb = slots + (size_t)(e >> 4) * (size_t)width * kSlotWords;
+            // parallel phase: lane s < ns loads, tests and (when far) expands slot s
+            float w[FWN_W_LINK];
+            uint32_t link = FWN_LINK_EMPTY;
+            float dx = 0.0f, dy = 0.0f, dz = 0.0f, d2 = 0.0f, t = 0.0f, inv3 = 0.0f;
+            bool far = false;
+            if (lane < ns) {
+                const uint4* src = reinterpret_cast<const uint4*>(nb + (size_t)lane * kSlotWords);
+                uint32_t u[kSlotWords];
+#pragma unroll
+                for (int k = 0; k < kSlotWords / 4; ++k) {
+                    const uint4 v = src[k];
+                    u[4 * k] = v.x;
+                    u[4 * k + 1] = v.y;
+                    u[4 * k + 2] = v.z;
+                    u[4 * k + 3] = v.w;
+                }
+#pragma unroll
+                for (int k = 0; k < FWN_W_LINK; ++k) w[k] = __uint_as_float(u[k]);
+                link = u[FWN_W_LINK];
+                dx = w[FWN_W_PX] - qx;
+                dy = w[FWN_W_PY] - qy;
+                dz = w[FWN_W_PZ] - qz;
+                d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                far = d2 > w[FWN_W_THR];
+                // evaluated on every lane (used only where far): the lanes run in parallel, and
+                // an unconditional use keeps all six loads of the slot in one round trip
+                far_field_terms(LaneWords{w}, dx, dy, dz, d2, t, inv3);
+            }
+            const uint64_t farm = __ballot(far);
+            // near leaves: the FP64 rule's thresholds from the leaf's first triangle
+            const bool nleaf = lane < ns && !far && (link & FWN_LINK_LEAF) != 0u;
+            bool use64 = false;
+            if (nleaf && tau > 0.0f) {
+                const int32_t first = (int32_t)((link & ~FWN_LINK_LEAF) >> 3);
+                const int32_t tcnt = (int32_t)(link & 7u) + 1;
+                // words 10-11 of a 48-byte record: one 8-byte load
+                const float2 np2 = *reinterpret_cast<const float2*>(tris + (size_t)first * kTriWords + FWN_T_NEAR2);
+                const float nd = fmaf(dx, w[FWN_W_NX], fmaf(dy, w[FWN_W_NY], dz * w[FWN_W_NZ]));
+                use64 = (d2 < np2.x) & ((tcnt > 1) | (nd * nd < np2.y)); // no short circuit: one load
+            }
+            const uint64_t m64 = __ballot(use64);
+            // serial phase: the terms in slot order, children pushed in slot order
+            for (int s2 = 0; s2 < ns; ++s2) {
+                const uint32_t ls = (uint32_t)__builtin_amdgcn_readlane((int)link, s2);
+                if ((farm >> s2) & 1ull) {
+                    om = fmaf(readlane_f(t, s2), readlane_f(inv3, s2), om);
+                } else if (ls & FWN_LINK_LEAF) {
+                    const int32_t first = (int32_t)((ls & ~FWN_LINK_LEAF) >> 3);
+                    const int32_t tcnt = (int32_t)(ls & 7u) + 1;
+                    if ((m64 >> s2) & 1ull) {
+                        double s64 = 0.0;
+                        for (int32_t k = first; k < first + tcnt; ++k)
+                            s64 += tri_f64(reinterpret_cast<const float*>(tris + (size_t)k * kTriWords), qx, qy, qz);
+                        acc += (unsigned long long)__double2ll_rn(s64 * kFixScale);
+                    } else {
+                        for (int32_t k = first; k < first + tcnt; ++k)
+                            om += tri_f32(reinterpret_cast<const float*>(tris + (size_t)k * kTriWords), qx, qy, qz);
+                    }
+                } else {
+                    overflow |= sp >= WaveStack<NB>::kCap;
+                    S.push(sp, lane, ls, 0ull);
+                    ++sp;
+                }
+            }
+        }
+        if (overflow && lane == 0) flag[0] = 1;
+    }
+    const long long v = (long long)acc;
+    if (v != 0) om = (float)((double)om + (double)v * (1.0 / kFixScale));
+    if (lane == 0) {
+        omega[i] = om;
+        if (inside) inside[i] = ((double)om / (4.0 * 3.14159265358979323846) > 0.5) ? 1 : 0;
+    }
 }
 
 __global__ void k_zero3(int32_t* a)
@@ -668,7 +822,66 @@ void run_queries_pipelined(Engine& e, const float* Q, int64_t nQ, float* omega, 
     }
 }
 
+// Host calls of at most kPointArgs points on a single-mesh engine: one k_point launch (points
+// in the arguments, results into the pinned stage) and one wait.  Per-point calls record no
+// device time.
+void run_points(Engine& e, const float* Q, int64_t nQ, float* omega, uint8_t* inside, int banks)
+{
+    hipStream_t s = e.stream;
+    const size_t out_b = 16 + (sizeof(float) + 1) * (size_t)nQ; // flag, omega, inside
+    char* hs = host_stage(e, out_b);
+    int32_t* hflag = reinterpret_cast<int32_t*>(hs);
+    float* hom = reinterpret_cast<float*>(hs + 16);
+    uint8_t* hin = reinterpret_cast<uint8_t*>(hs + 16 + sizeof(float) * (size_t)nQ);
+    *hflag = 0;
+    PointArgs pa;
+    std::memset(&pa, 0, sizeof(pa));
+    std::memcpy(pa.q, Q, sizeof(float) * 3 * (size_t)nQ);
+    const int32_t root = e.h_roots[0], root_ns = e.h_roots[1];
+    const float tau = e.h_tau[0];
+    if (banks == 1)
+        hipLaunchKernelGGL(k_point<1>, dim3((unsigned)nQ), dim3(64), 0, s, e.slots.as<uint32_t>(), e.width,
+                           e.tris.as<uint32_t>(), root, root_ns, tau, pa, hom, hin, hflag);
+    else
+        hipLaunchKernelGGL(k_point<3>, dim3((unsigned)nQ), dim3(64), 0, s, e.slots.as<uint32_t>(), e.width,
+                           e.tris.as<uint32_t>(), root, root_ns, tau, pa, hom, hin, hflag);
+    FWN_LAUNCH_CHECK("k_point");
+    FWN_HIP(hipStreamSynchronize(s));
+    if (*hflag) fail(FWN_LIMIT_EXCEEDED, "traversal stack overflow");
+    std::memcpy(omega, hom, sizeof(float) * (size_t)nQ);
+    if (inside) std::memcpy(inside, hin, (size_t)nQ);
+    e.last_traverse_ms = 0.0;
+    e.last_total_ms = 0.0;
+}
+
 } // namespace
+
+void launch_point_keys(Engine& e, const float* dQ, int64_t nQ, uint32_t* keys, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_query_keys, dim3(grid_blocks(nQ, QT)), dim3(QT), 0, s, dQ, nQ, (const int64_t*)nullptr, 1, 0,
+                       e.mesh_info.as<float>(), keys);
+    FWN_LAUNCH_CHECK("k_query_keys");
+}
+
+void launch_traverse_inplace(Engine& e, int banks, const float* dQ, const uint32_t* perm, const int64_t* meta,
+                             int64_t nQ, float* omega, uint8_t* inside, int32_t* flag, hipStream_t s)
+{
+    const int64_t nb = (nQ + 63) / 64;
+    if (nb > INT32_MAX) fail(FWN_LIMIT_EXCEEDED, "too many point batches");
+    const float beta2 = e.opt.accuracy_scale * e.opt.accuracy_scale;
+    const int64_t* q_off = meta;
+    const int64_t* b_off = meta + 2;
+    const int32_t* order = reinterpret_cast<const int32_t*>(meta + 4);
+#define FWN_TRAV_INL(NB)                                                                                            \
+    hipLaunchKernelGGL((k_traverse<false, 1, false, NB, true>), dim3((unsigned)nb), dim3(64), 0, s,                \
+                       e.slots.as<uint32_t>(), e.width, e.tris.as<uint32_t>(), e.roots.as<int32_t>(),              \
+                       e.mesh_info.as<float>(), dQ, perm, q_off, b_off, order, 1, nb, 64, beta2, omega, inside,    \
+                       (int32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr, 0u, flag, (uint32_t*)nullptr)
+    if (banks == 1) FWN_TRAV_INL(1);
+    else FWN_TRAV_INL(3);
+#undef FWN_TRAV_INL
+    FWN_LAUNCH_CHECK("k_traverse");
+}
 
 void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t nQ, float* omega, uint8_t* inside,
                  int32_t* stats, bool dev)
@@ -694,6 +907,10 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
     if (banks == 0) fail(FWN_LIMIT_EXCEEDED, "tree deeper than the traversal stack supports");
     if (!dev && !stats && M == 1 && nQ > kPipeChunk) {
         run_queries_pipelined(e, Q, nQ, omega, inside, banks);
+        return;
+    }
+    if (!dev && !stats && M == 1 && nQ <= kPointArgs) {
+        run_points(e, Q, nQ, omega, inside, banks);
         return;
     }
 

@@ -323,9 +323,11 @@ __global__ __launch_bounds__(256) void k_iota(uint32_t* __restrict__ v, int64_t 
     if (i < n) v[i] = (uint32_t)i;
 }
 
+// the passes over caller-provided histogram scratch: hist holds 256 x ceil(n / RS_TILE) and
+// rowsum 256 int32
 template <typename KeyT>
-void radix_sort_impl(KeyT*& keys, uint32_t*& vals, KeyT*& keys_alt, uint32_t*& vals_alt, int64_t n, int begin_bit,
-                     int end_bit, bool vals_are_index, DBuf& hist, DBuf& rowsum, hipStream_t s)
+void radix_sort_core(KeyT*& keys, uint32_t*& vals, KeyT*& keys_alt, uint32_t*& vals_alt, int64_t n, int begin_bit,
+                     int end_bit, bool vals_are_index, int32_t* hist, int32_t* rowsum, hipStream_t s)
 {
     if (n <= 1 || begin_bit >= end_bit) {
         if (n >= 1 && vals_are_index) {
@@ -335,25 +337,34 @@ void radix_sort_impl(KeyT*& keys, uint32_t*& vals, KeyT*& keys_alt, uint32_t*& v
         return;
     }
     if (n > (int64_t)INT32_MAX) fail(FWN_LIMIT_EXCEEDED, "radix sort: more than 2^31-1 elements");
-    int nblocks = (int)((n + RS_TILE - 1) / RS_TILE);
-    hist.reserve(sizeof(int32_t) * 256 * (size_t)nblocks);
-    rowsum.reserve(sizeof(int32_t) * 256);
+    const int nblocks = (int)((n + RS_TILE - 1) / RS_TILE);
     for (int shift = begin_bit; shift < end_bit; shift += 8) {
-        hipLaunchKernelGGL(k_rs_hist<KeyT>, dim3(nblocks), dim3(RS_THREADS), 0, s, keys, n, shift,
-                           hist.as<int32_t>(), nblocks);
+        hipLaunchKernelGGL(k_rs_hist<KeyT>, dim3(nblocks), dim3(RS_THREADS), 0, s, keys, n, shift, hist, nblocks);
         FWN_LAUNCH_CHECK("k_rs_hist");
-        hipLaunchKernelGGL(k_rs_rows, dim3(256), dim3(256), 0, s, hist.as<int32_t>(), nblocks,
-                           rowsum.as<int32_t>());
+        hipLaunchKernelGGL(k_rs_rows, dim3(256), dim3(256), 0, s, hist, nblocks, rowsum);
         FWN_LAUNCH_CHECK("k_rs_rows");
-        hipLaunchKernelGGL(k_rs_digits, dim3(1), dim3(256), 0, s, rowsum.as<int32_t>());
+        hipLaunchKernelGGL(k_rs_digits, dim3(1), dim3(256), 0, s, rowsum);
         FWN_LAUNCH_CHECK("k_rs_digits");
         const uint32_t* vin = (vals_are_index && shift == begin_bit) ? nullptr : vals;
         hipLaunchKernelGGL(k_rs_scatter<KeyT>, dim3(nblocks), dim3(RS_THREADS), 0, s, keys, vin, keys_alt,
-                           vals_alt, n, shift, hist.as<int32_t>(), rowsum.as<int32_t>(), nblocks);
+                           vals_alt, n, shift, hist, rowsum, nblocks);
         FWN_LAUNCH_CHECK("k_rs_scatter");
         std::swap(keys, keys_alt);
         std::swap(vals, vals_alt);
     }
+}
+
+template <typename KeyT>
+void radix_sort_impl(KeyT*& keys, uint32_t*& vals, KeyT*& keys_alt, uint32_t*& vals_alt, int64_t n, int begin_bit,
+                     int end_bit, bool vals_are_index, DBuf& hist, DBuf& rowsum, hipStream_t s)
+{
+    if (n > 1 && begin_bit < end_bit) {
+        const int64_t nblocks = (n + RS_TILE - 1) / RS_TILE;
+        hist.reserve(sizeof(int32_t) * 256 * (size_t)nblocks);
+        rowsum.reserve(sizeof(int32_t) * 256);
+    }
+    radix_sort_core<KeyT>(keys, vals, keys_alt, vals_alt, n, begin_bit, end_bit, vals_are_index, hist.as<int32_t>(),
+                          rowsum.as<int32_t>(), s);
 }
 
 } // namespace
@@ -368,6 +379,18 @@ void radix_sort_pairs32(uint32_t*& keys, uint32_t*& vals, uint32_t*& keys_alt, u
                         int begin_bit, int end_bit, bool vals_are_index, DBuf& hist, DBuf& rowsum, hipStream_t s)
 {
     radix_sort_impl<uint32_t>(keys, vals, keys_alt, vals_alt, n, begin_bit, end_bit, vals_are_index, hist, rowsum, s);
+}
+
+int64_t radix_sort_hist_words(int64_t n)
+{
+    return 256 * ((n + RS_TILE - 1) / RS_TILE);
+}
+
+void radix_sort_pairs32_scratch(uint32_t*& keys, uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt, int64_t n,
+                                int begin_bit, int end_bit, bool vals_are_index, int32_t* hist, int32_t* rowsum,
+                                hipStream_t s)
+{
+    radix_sort_core<uint32_t>(keys, vals, keys_alt, vals_alt, n, begin_bit, end_bit, vals_are_index, hist, rowsum, s);
 }
 
 // out must hold n+1 entries; out[n] receives the total.

@@ -265,30 +265,31 @@ API void oracle_moments(const float* V, const int32_t* F, const int64_t* tris, i
     free(tm);
 }
 
-/* record -> 32-word slot (the engine's format, fwn.h) */
-static void slot_from_moments(const double* m, uint32_t* w)
+/* record -> slot words 0..22 (the engine's format, fwn.h); the threshold word is written by
+ * set_thresholds from *r2out (the squared radius bound, rounded up to float) */
+static void slot_from_moments(const double* m, uint32_t* w, float* r2out)
 {
-    float f[27];
-    f[0] = (float)m[MP];
-    f[1] = (float)m[MP + 1];
-    f[2] = (float)m[MP + 2];
+    float f[FWN_W_LINK];
+    f[FWN_W_PX] = (float)m[MP];
+    f[FWN_W_PY] = (float)m[MP + 1];
+    f[FWN_W_PZ] = (float)m[MP + 2];
+    f[FWN_W_THR] = 0.0f;
     /* round r2 up so the float radius stays an upper bound */
-    f[3] = nextafterf((float)m[MR2], INFINITY);
-    f[4] = (float)m[MN];
-    f[5] = (float)m[MN + 1];
-    f[6] = (float)m[MN + 2];
+    *r2out = nextafterf((float)m[MR2], INFINITY);
+    f[FWN_W_NX] = (float)m[MN];
+    f[FWN_W_NY] = (float)m[MN + 1];
+    f[FWN_W_NZ] = (float)m[MN + 2];
     /* tr joins the diagonal of the quadratic (tr = tr |d|^2 / |d|^2) and C the cubic monomials
-     * d_i d_j d_j ((C.d) = (C.d) |d|^2 / |d|^2), so the far field has no constant terms; words
-     * 7 and 24-26 keep tr and C themselves */
+     * d_i d_j d_j ((C.d) = (C.d) |d|^2 / |d|^2), so the far field has no constant terms */
     const double* M = m + M1;
     const double tr = M[0] + M[4] + M[8];
-    f[7] = (float)tr;
-    f[8] = (float)(-3.0 * M[0] + tr);
-    f[9] = (float)(-3.0 * M[4] + tr);
-    f[10] = (float)(-3.0 * M[8] + tr);
-    f[11] = (float)(-3.0 * (M[1] + M[3]));
-    f[12] = (float)(-3.0 * (M[5] + M[7]));
-    f[13] = (float)(-3.0 * (M[6] + M[2]));
+    float* A = f + FWN_W_A;
+    A[0] = (float)(-3.0 * M[0] + tr);
+    A[1] = (float)(-3.0 * M[4] + tr);
+    A[2] = (float)(-3.0 * M[8] + tr);
+    A[3] = (float)(-3.0 * (M[1] + M[3]));
+    A[4] = (float)(-3.0 * (M[5] + M[7]));
+    A[5] = (float)(-3.0 * (M[6] + M[2]));
     const double* S = m + MS;
     static const double sc[10] = {7.5, 7.5, 7.5, 22.5, 22.5, 22.5, 22.5, 22.5, 22.5, 45.0};
     const double cx = -4.5 * (S[0] + S[5] + S[7]);
@@ -296,11 +297,40 @@ static void slot_from_moments(const double* m, uint32_t* w)
     const double cz = -4.5 * (S[4] + S[6] + S[2]);
     /* monomials: xxx yyy zzz xxy xxz yyx yyz zzx zzy xyz */
     const double fold[10] = {cx, cy, cz, cy, cz, cx, cz, cx, cy, 0.0};
-    for (int s = 0; s < 10; ++s) f[14 + s] = (float)(sc[s] * S[s] + fold[s]);
-    f[24] = (float)cx;
-    f[25] = (float)cy;
-    f[26] = (float)cz;
+    for (int s = 0; s < 10; ++s) f[FWN_W_B + s] = (float)(sc[s] * S[s] + fold[s]);
     memcpy(w, f, sizeof(f));
+}
+
+static inline uint32_t leaf_link(int64_t first, int64_t count)
+{
+    return FWN_LINK_LEAF | ((uint32_t)first << 3) | (uint32_t)(count - 1);
+}
+
+/* Far-field thresholds and leaf near-field data of a slot tree for accuracy scale beta and FP64
+ * distance tau, from each slot's squared radius bound r2 (float): thr = (beta^2) r2, and for a
+ * leaf (first triangle t) tris[t] words 10-11 = (sqrt(r2) + tau)^2 and tau^2 |N|^2, the same
+ * float operations as the engine's build. */
+API void set_thresholds(uint32_t* slots, int64_t nslots, uint32_t* tris, const float* r2, float beta, float tau)
+{
+    const float beta2 = beta * beta;
+    for (int64_t k = 0; k < nslots; ++k) {
+        uint32_t* w = slots + k * FWN_SLOT_WORDS;
+        const uint32_t link = w[FWN_W_LINK];
+        if (link == FWN_LINK_EMPTY) continue;
+        float f[FWN_W_LINK];
+        memcpy(f, w, sizeof(f));
+        const float thr = beta2 * r2[k];
+        memcpy(w + FWN_W_THR, &thr, 4);
+        if (link & FWN_LINK_LEAF) {
+            const int64_t first = (int64_t)((link & ~FWN_LINK_LEAF) >> 3);
+            const float rr = sqrtf(r2[k]) + tau;
+            const float near2 = rr * rr;
+            const float nn = fmaf(f[FWN_W_NX], f[FWN_W_NX], fmaf(f[FWN_W_NY], f[FWN_W_NY], f[FWN_W_NZ] * f[FWN_W_NZ]));
+            const float plane = (tau * tau) * nn;
+            memcpy(tris + first * FWN_TRI_WORDS + FWN_T_NEAR2, &near2, 4);
+            memcpy(tris + first * FWN_TRI_WORDS + FWN_T_PLANE, &plane, 4);
+        }
+    }
 }
 
 /* ------------------------------------------------------------------------------------ */
@@ -309,8 +339,9 @@ static void slot_from_moments(const double* m, uint32_t* w)
 typedef struct restated_tree {
     int64_t nnodes, cap;
     int32_t width;   /* slots per node: 4 restates the reference; 2 and 8 are for study */
-    uint32_t* slots; /* nnodes * width * 32 words */
-    uint32_t* tris;  /* nF * 12 words, in tree order */
+    uint32_t* slots; /* nnodes * width * FWN_SLOT_WORDS words */
+    float* r2;       /* nnodes * width squared radius bounds (thresholds: set_thresholds) */
+    uint32_t* tris;  /* nF * FWN_TRI_WORDS words, in tree order */
     int64_t nF;
     int32_t root_ns;
     int32_t depth;
@@ -419,9 +450,15 @@ static int64_t rnode_alloc(restated_tree* T)
     if (T->nnodes == T->cap) {
         T->cap = T->cap ? 2 * T->cap : 1024;
         T->slots = (uint32_t*)realloc(T->slots, sizeof(uint32_t) * T->width * FWN_SLOT_WORDS * (size_t)T->cap);
+        T->r2 = (float*)realloc(T->r2, sizeof(float) * T->width * (size_t)T->cap);
     }
     int64_t id = T->nnodes++;
-    memset(T->slots + T->width * FWN_SLOT_WORDS * id, 0, sizeof(uint32_t) * T->width * FWN_SLOT_WORDS);
+    uint32_t* w = T->slots + T->width * FWN_SLOT_WORDS * id;
+    memset(w, 0, sizeof(uint32_t) * T->width * FWN_SLOT_WORDS);
+    for (int k = 0; k < T->width; ++k) {
+        w[k * FWN_SLOT_WORDS + FWN_W_LINK] = FWN_LINK_EMPTY;
+        T->r2[T->width * id + k] = 0.0f;
+    }
     return id;
 }
 
@@ -466,8 +503,8 @@ static int rbuild_node(rbuild* B, int64_t lo, int64_t hi, int64_t node, double* 
                     v[k][d] = B->V[3 * (int64_t)f[k] + d];
                     memcpy(tw + 3 * k + d, &B->V[3 * (int64_t)f[k] + d], 4);
                 }
-            tw[9] = (uint32_t)t;
-            tw[10] = tw[11] = 0;
+            tw[FWN_T_FACET] = (uint32_t)t;
+            tw[FWN_T_NEAR2] = tw[FWN_T_PLANE] = 0;
             B->tri_out++;
             tri_moments(v[0], v[1], v[2], rec[s]);
         } else {
@@ -480,11 +517,10 @@ static int rbuild_node(rbuild* B, int64_t lo, int64_t hi, int64_t node, double* 
             uint32_t* cw = B->T->slots + (int64_t)width * cn * FWN_SLOT_WORDS;
             child_ns = 0;
             for (int k = 0; k < width; ++k)
-                if (cw[k * FWN_SLOT_WORDS + FWN_W_COUNT] != 0) child_ns++;
+                if (cw[k * FWN_SLOT_WORDS + FWN_W_LINK] != FWN_LINK_EMPTY) child_ns++;
         }
-        slot_from_moments(rec[s], w);
-        int32_t ints[5] = {child, (int32_t)first, (int32_t)(b - a), child_ns, 0};
-        memcpy(w + FWN_W_CHILD, ints, sizeof(ints));
+        slot_from_moments(rec[s], w, B->T->r2 + (int64_t)width * node + s);
+        w[FWN_W_LINK] = child >= 0 ? (((uint32_t)child << 4) | (uint32_t)child_ns) : leaf_link(first, b - a);
         chp[s] = rec[s];
     }
     merge_moments(chp, ng, mout);
@@ -531,14 +567,14 @@ API restated_tree* restated_build_w(const float* V, int64_t nV, const int32_t* F
                 memcpy(T->tris + 3 * k + d, &V[3 * (int64_t)f[k] + d], 4);
             }
         tri_moments(v[0], v[1], v[2], rec);
-        slot_from_moments(rec, T->slots);
-        int32_t ints[5] = {-1, 0, 1, 0, 0};
-        memcpy(T->slots + FWN_W_CHILD, ints, sizeof(ints));
+        slot_from_moments(rec, T->slots, T->r2);
+        T->slots[FWN_W_LINK] = leaf_link(0, 1);
+        T->tris[FWN_T_FACET] = 0;
         T->depth = 1;
     }
     T->root_ns = 0;
     for (int k = 0; k < T->width; ++k)
-        if (T->slots[k * FWN_SLOT_WORDS + FWN_W_COUNT] != 0) T->root_ns++;
+        if (T->slots[k * FWN_SLOT_WORDS + FWN_W_LINK] != FWN_LINK_EMPTY) T->root_ns++;
     free(B.cen);
     free(B.blo);
     free(B.bhi);
@@ -555,6 +591,7 @@ API void restated_free(restated_tree* T)
 {
     if (!T) return;
     free(T->slots);
+    free(T->r2);
     free(T->tris);
     free(T);
 }
@@ -564,6 +601,7 @@ API int32_t restated_root_ns(const restated_tree* T) { return T->root_ns; }
 API int32_t restated_depth(const restated_tree* T) { return T->depth; }
 API const uint32_t* restated_slots(const restated_tree* T) { return T->slots; }
 API const uint32_t* restated_tris(const restated_tree* T) { return T->tris; }
+API const float* restated_r2(const restated_tree* T) { return T->r2; }
 
 /* ------------------------------------------------------------------------------------ */
 /* FP32 traversal (restated engine and same-tree check of the GPU engine)                 */
@@ -635,14 +673,15 @@ static inline float far_field_acc(const float* w, float dx, float dy, float dz, 
 typedef struct { int32_t node; int32_t ns; } tstack;
 
 /* One point over a slot tree of the given width.  Children are pushed in slot order and popped
- * last-in-first-out, which is the order the GPU kernel sums a point's FP32 terms in.  With
- * tau > 0, a near leaf's triangles are taken in FP64 when q lies within tau of the leaf's
- * bounding sphere (sqrt(r2) + tau) and, for a single-triangle leaf, within tau of its plane;
- * each such leaf's FP64 sum is rounded to 2^-40 fixed point and the fixed-point total is added
- * to the FP32 sum once, at the end (the engine's deferred near-field pass does the same). */
+ * last-in-first-out, which is the order the GPU kernel sums a point's FP32 terms in.  A slot is
+ * far when |p - q|^2 exceeds its threshold word (beta^2 r2, set_thresholds).  With tau > 0, a
+ * near leaf's triangles are taken in FP64 when q lies within tau of the leaf's bounding sphere
+ * (|d|^2 < (sqrt(r2) + tau)^2) and, for a single-triangle leaf, within tau of its plane
+ * ((d.N)^2 < tau^2 |N|^2), both thresholds read from the leaf's first triangle; each such leaf's
+ * FP64 sum is rounded to 2^-40 fixed point and the fixed-point total is added to the FP32 sum
+ * once, at the end (the engine's deferred near-field pass does the same). */
 static float eval_point(const uint32_t* slots, int width, int32_t root, int32_t root_ns,
-                        const uint32_t* tris, float beta2, float tau, float qx, float qy, float qz,
-                        int32_t* st)
+                        const uint32_t* tris, float tau, float qx, float qy, float qz, int32_t* st)
 {
     tstack stack[1024];
     int sp = 0;
@@ -651,26 +690,24 @@ static float eval_point(const uint32_t* slots, int width, int32_t root, int32_t 
     sp++;
     float om = 0.0f;
     long long acc = 0;
-    const float tau2 = tau * tau;
     int32_t nt = 0, nf = 0, n32 = 0, n64 = 0;
     while (sp > 0) {
         --sp;
         int32_t node = stack[sp].node, ns = stack[sp].ns;
         for (int s = 0; s < ns; ++s) {
             const float* w = (const float*)(slots + ((int64_t)node * width + s) * FWN_SLOT_WORDS);
-            const int32_t* wi = (const int32_t*)w;
+            const uint32_t link = slots[((int64_t)node * width + s) * FWN_SLOT_WORDS + FWN_W_LINK];
             float dx = w[FWN_W_PX] - qx, dy = w[FWN_W_PY] - qy, dz = w[FWN_W_PZ] - qz;
             float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
             nt++;
-            if (d2 > beta2 * w[FWN_W_R2]) {
+            if (d2 > w[FWN_W_THR]) {
                 om = far_field_acc(w, dx, dy, dz, d2, om);
                 nf++;
-            } else if (wi[FWN_W_CHILD] < 0) {
-                int32_t first = wi[FWN_W_FIRST], cnt = wi[FWN_W_COUNT];
-                float rr = sqrtf(w[FWN_W_R2]) + tau;
+            } else if (link & FWN_LINK_LEAF) {
+                int32_t first = (int32_t)((link & ~FWN_LINK_LEAF) >> 3), cnt = (int32_t)(link & 7u) + 1;
+                const float* t0 = (const float*)(tris + (int64_t)first * FWN_TRI_WORDS);
                 float nd = fmaf(dx, w[FWN_W_NX], fmaf(dy, w[FWN_W_NY], dz * w[FWN_W_NZ]));
-                float nn = fmaf(w[FWN_W_NX], w[FWN_W_NX], fmaf(w[FWN_W_NY], w[FWN_W_NY], w[FWN_W_NZ] * w[FWN_W_NZ]));
-                int use64 = tau > 0.0f && d2 < rr * rr && (cnt > 1 || nd * nd < tau2 * nn);
+                int use64 = tau > 0.0f && d2 < t0[FWN_T_NEAR2] && (cnt > 1 || nd * nd < t0[FWN_T_PLANE]);
                 if (use64) {
                     double s64 = 0.0;
                     for (int32_t t = first; t < first + cnt; ++t)
@@ -685,8 +722,8 @@ static float eval_point(const uint32_t* slots, int width, int32_t root, int32_t 
                 }
             } else {
                 if (sp >= 1024) return NAN;
-                stack[sp].node = wi[FWN_W_CHILD];
-                stack[sp].ns = wi[FWN_W_CHILD_NS];
+                stack[sp].node = (int32_t)(link >> 4);
+                stack[sp].ns = (int32_t)(link & 15u);
                 sp++;
             }
         }
@@ -702,15 +739,14 @@ static float eval_point(const uint32_t* slots, int width, int32_t root, int32_t 
 }
 
 API void tree_eval(const uint32_t* slots, int32_t width, int32_t root, int32_t root_ns, const uint32_t* tris,
-                   const float* Q, int64_t nQ, float beta, float tau, float* out, int32_t* stats, int nthreads)
+                   const float* Q, int64_t nQ, float tau, float* out, int32_t* stats, int nthreads)
 {
-    float beta2 = beta * beta;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 256)
 #endif
     for (int64_t i = 0; i < nQ; ++i)
-        out[i] = eval_point(slots, width, root, root_ns, tris, beta2, tau, Q[3 * i], Q[3 * i + 1], Q[3 * i + 2],
+        out[i] = eval_point(slots, width, root, root_ns, tris, tau, Q[3 * i], Q[3 * i + 1], Q[3 * i + 2],
                             stats ? stats + 4 * i : NULL);
 }
 

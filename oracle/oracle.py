@@ -29,7 +29,10 @@ U64P = C.POINTER(C.c_uint64)
 U8P = C.POINTER(C.c_uint8)
 
 WIDTH_RESTATED = 4
-SLOT_WORDS = 32
+SLOT_WORDS = 24
+LINK = 23        # slot word: (child << 4) | child slots, or LINK_LEAF | first << 3 | count - 1
+LINK_LEAF = 0x80000000
+LINK_EMPTY = 0xFFFFFFFF
 TRI_WORDS = 12
 
 
@@ -66,8 +69,11 @@ def lib():
         L.restated_slots.argtypes = [C.c_void_p]
         L.restated_tris.restype = U32P
         L.restated_tris.argtypes = [C.c_void_p]
-        L.tree_eval.argtypes = [U32P, C.c_int32, C.c_int32, C.c_int32, U32P, FP, C.c_int64, C.c_float,
-                                C.c_float, FP, I32P, C.c_int]
+        L.restated_r2.restype = FP
+        L.restated_r2.argtypes = [C.c_void_p]
+        L.set_thresholds.argtypes = [U32P, C.c_int64, U32P, FP, C.c_float, C.c_float]
+        L.tree_eval.argtypes = [U32P, C.c_int32, C.c_int32, C.c_int32, U32P, FP, C.c_int64, C.c_float, FP,
+                                I32P, C.c_int]
         L.inside_from_omega.argtypes = [FP, C.c_int64, U8P]
         L.lbvh_sorted_order.argtypes = [C.c_int32, FP, I64P, I32P, I64P, I64P, U64P]
         L.oracle_tri_closest.restype = C.c_double
@@ -158,21 +164,103 @@ def inside_from_omega(om) -> np.ndarray:
     return out.astype(bool)
 
 
-def tree_eval(slots, width, root, root_ns, tris, Q, beta=2.0, tau=0.0, stats=False, nthreads=0):
+def tree_eval(slots, width, root, root_ns, tris, Q, tau=0.0, stats=False, nthreads=0):
+    """FP32 traversal of a slot tree (the engine's exported format, or a Restated tree's): each
+    slot's far-field threshold and each leaf's FP64 near-field thresholds are read from the
+    arrays; tau > 0 enables the FP64 near field."""
     slots = np.ascontiguousarray(slots, dtype=np.uint32)
     tris = np.ascontiguousarray(tris, dtype=np.uint32)
     Q = _f32(Q)
     out = np.empty(Q.shape[0], np.float32)
     st = np.empty((Q.shape[0], 4), np.int32) if stats else None
-    lib().tree_eval(_p(slots, U32P), width, root, root_ns, _p(tris, U32P), _p(Q, FP), Q.shape[0], beta, tau,
+    lib().tree_eval(_p(slots, U32P), width, root, root_ns, _p(tris, U32P), _p(Q, FP), Q.shape[0], tau,
                     _p(out, FP), _p(st, I32P) if stats else None, nthreads)
     return (out, st) if stats else out
 
 
-class Restated:
-    """CPU restatement of the reference engine (FP32, order 2, beta = 2 by default)."""
+def decode_links(slots):
+    """Per slot of a slot array: (is_leaf, child node, child slot count, first triangle, count,
+    empty), decoded from the link word."""
+    link = np.ascontiguousarray(slots, dtype=np.uint32).reshape(-1, SLOT_WORDS)[:, LINK].astype(np.int64)
+    empty = link == LINK_EMPTY
+    leaf = ((link & LINK_LEAF) != 0) & ~empty
+    inner = ~leaf & ~empty
+    child = np.where(inner, link >> 4, -1)
+    child_ns = np.where(inner, link & 15, 0)
+    first = np.where(leaf, (link & 0x7FFFFFFF) >> 3, -1)
+    count = np.where(leaf, (link & 7) + 1, 0)
+    return leaf, child, child_ns, first, count, empty
 
-    def __init__(self, V, F, width: int = WIDTH_RESTATED):
+
+def slot_ranges(slots, width, roots):
+    """Triangle range (first, count) covered by every slot of a slot tree, from its leaves
+    (internal slots store only their child link).  roots: (root node, root slot count) pairs."""
+    leaf, child, child_ns, first, count, empty = decode_links(slots)
+    n = first.shape[0]
+    first = first.copy()
+    count = count.copy()
+    # level order from the roots, then internal slots take the union of their child's slots
+    levels = []
+    cur = np.array([r for r, ns in roots if r >= 0], np.int64)
+    while cur.size:
+        levels.append(cur)
+        sl = (cur[:, None] * width + np.arange(width)[None, :]).ravel()
+        sl = sl[sl < n]
+        nxt = child[sl]
+        cur = np.unique(nxt[nxt >= 0])
+    for nodes in reversed(levels):
+        sl = (nodes[:, None] * width + np.arange(width)[None, :]).ravel()
+        inner = sl[child[sl] >= 0]
+        cs = child[inner][:, None] * width + np.arange(width)[None, :]
+        occ = ~empty[cs]
+        cf = np.where(occ, first[cs], np.iinfo(np.int64).max)
+        first[inner] = cf.min(axis=1)
+        count[inner] = np.where(occ, count[cs], 0).sum(axis=1)
+    return first, count
+
+
+def compare_slot_trees(gs, groot, gns, rs, rns, width, rtol=2e-5):
+    """Walks two slot trees together, one level at a time: every slot's leaf / internal kind,
+    leaf triangle range and child slot count equal, float words within rtol of the slot's
+    largest word (node numbering may differ).  Returns the number of nodes visited."""
+    gs = np.ascontiguousarray(gs, np.uint32).reshape(-1, SLOT_WORDS)
+    rs = np.ascontiguousarray(rs, np.uint32).reshape(-1, SLOT_WORDS)
+    gf, rf = gs.view(np.float32), rs.view(np.float32)
+    gl, gc, gcn, gfi, gco, _ = decode_links(gs)
+    rl, rc, rcn, rfi, rco, _ = decode_links(rs)
+    assert gns == rns, (gns, rns)
+    g = np.array([groot], np.int64)
+    r = np.array([0], np.int64)
+    ns = np.array([gns], np.int64)
+    nodes = 0
+    while g.size:
+        nodes += g.size
+        nxt_g, nxt_r, nxt_ns = [], [], []
+        for k in range(width):
+            sel = ns > k
+            a, b = g[sel] * width + k, r[sel] * width + k
+            np.testing.assert_array_equal(gl[a], rl[b])
+            np.testing.assert_array_equal(gfi[a], rfi[b])
+            np.testing.assert_array_equal(gco[a], rco[b])
+            np.testing.assert_array_equal(gcn[a], rcn[b])
+            gv, rv = gf[a, :LINK].astype(np.float64), rf[b, :LINK].astype(np.float64)
+            scale = np.maximum(np.abs(rv).max(axis=1), 1e-30)
+            bad = np.abs(gv - rv).max(axis=1) > rtol * scale
+            assert not bad.any(), (a[bad][:5], (np.abs(gv - rv).max(axis=1) / scale)[bad][:5])
+            inner = ~gl[a]
+            nxt_g.append(gc[a][inner])
+            nxt_r.append(rc[b][inner])
+            nxt_ns.append(gcn[a][inner])
+        g, r, ns = (np.concatenate(x).astype(np.int64) for x in (nxt_g, nxt_r, nxt_ns))
+    return nodes
+
+
+class Restated:
+    """CPU restatement of the reference engine (FP32, order 2, beta = 2 by default).  `slots` and
+    `tris` carry the thresholds of accuracy scale `beta` and FP64 distance `tau` (0: FP32 only,
+    the reference engine); `thresholds()` gives them for other values."""
+
+    def __init__(self, V, F, width: int = WIDTH_RESTATED, beta: float = 2.0, tau: float = 0.0):
         """width 4 restates the reference's 4-ary tree; 2 and 8 exist for comparison."""
         self.V, self.F = _f32(V), _i32(F)
         self.width = int(width)
@@ -182,15 +270,32 @@ class Restated:
         self.num_nodes = int(n)
         self.root_ns = int(L.restated_root_ns(self._h))
         self.depth = int(L.restated_depth(self._h))
-        self.slots = np.ctypeslib.as_array(L.restated_slots(self._h),
-                                           shape=(max(n, 1) * self.width * SLOT_WORDS,)).copy()
-        self.tris = np.ctypeslib.as_array(L.restated_tris(self._h),
-                                          shape=(max(self.F.shape[0], 1) * TRI_WORDS,)).copy()
+        nslots = max(n, 1) * self.width
+        self._slots0 = np.ctypeslib.as_array(L.restated_slots(self._h), shape=(nslots * SLOT_WORDS,)).copy()
+        self._tris0 = np.ctypeslib.as_array(L.restated_tris(self._h),
+                                            shape=(max(self.F.shape[0], 1) * TRI_WORDS,)).copy()
+        self.r2 = np.ctypeslib.as_array(L.restated_r2(self._h), shape=(nslots,)).copy() if n > 0 else \
+            np.zeros(nslots, np.float32)
         L.restated_free(self._h)
         self._h = None
+        self.beta, self.tau = float(beta), float(tau)
+        self.slots, self.tris = self.thresholds(self.beta, self.tau)
+        self._cache = {(self.beta, self.tau): (self.slots, self.tris)}
 
-    def solid_angle(self, Q, beta: float = 2.0, stats: bool = False, nthreads: int = 0):
-        return tree_eval(self.slots, self.width, 0, self.root_ns, self.tris, Q, beta, 0.0, stats, nthreads)
+    def thresholds(self, beta: float, tau: float = 0.0):
+        """(slots, tris) copies with the far-field thresholds of accuracy scale beta and the
+        leaves' FP64 near-field thresholds of distance tau."""
+        slots, tris = self._slots0.copy(), self._tris0.copy()
+        r2 = np.ascontiguousarray(self.r2, np.float32)
+        lib().set_thresholds(_p(slots, U32P), r2.shape[0], _p(tris, U32P), _p(r2, FP), beta, tau)
+        return slots, tris
+
+    def solid_angle(self, Q, beta: float = 2.0, stats: bool = False, nthreads: int = 0, tau: float = 0.0):
+        key = (float(beta), float(tau))
+        if key not in self._cache:
+            self._cache[key] = self.thresholds(*key)
+        slots, tris = self._cache[key]
+        return tree_eval(slots, self.width, 0, self.root_ns, tris, Q, tau, stats, nthreads)
 
     def is_inside(self, Q, beta: float = 2.0):
         return inside_from_omega(self.solid_angle(Q, beta))

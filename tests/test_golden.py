@@ -43,7 +43,7 @@ def test_gpu_engine_on_golden(oracle, golden, name):
         assert err.max() <= 1e-6
     R = oracle.Restated(V, F)
     tau = float(e.export_tree()[3][0])
-    ref = oracle.tree_eval(R.slots, 4, 0, R.root_ns, R.tris, Q, 2.0, tau)
+    ref = R.solid_angle(Q, 2.0, tau=tau)
     assert np.abs(got.astype(np.float64) - ref).max() / (4 * np.pi) <= 2e-5
     np.testing.assert_allclose(e.bruteforce_f64(Q), om, rtol=0, atol=1e-9)
     w = om / (4 * np.pi)

@@ -49,36 +49,6 @@ def test_c1_icosphere_million_points(oracle):
     assert ins[r < 0.99].all() and not ins[r > 1.01].any()
 
 
-def _compare_trees_levelwise(gs, groot, gns, rs, rns, width=4):
-    """Walks the GPU tree and the restated tree together, one level at a time (vectorised):
-    every slot's covered range, leaf / internal flag and child slot count equal, float words
-    within 2e-5 of the slot's largest word.  Returns the number of nodes visited."""
-    gf, gi = gs.reshape(-1, 32).view(np.float32), gs.reshape(-1, 32).view(np.int32)
-    rf, ri = rs.reshape(-1, 32).view(np.float32), rs.reshape(-1, 32).view(np.int32)
-    g = np.array([groot], np.int64)
-    r = np.array([0], np.int64)
-    ns = np.array([gns], np.int64)
-    assert gns == rns
-    nodes = 0
-    while g.size:
-        nodes += g.size
-        nxt_g, nxt_r, nxt_ns = [], [], []
-        for k in range(width):
-            sel = ns > k
-            a, b = g[sel] * width + k, r[sel] * width + k
-            np.testing.assert_array_equal(gi[a, 28:31], ri[b, 28:31])
-            np.testing.assert_array_equal(gi[a, 27] < 0, ri[b, 27] < 0)
-            gv, rv = gf[a, :27].astype(np.float64), rf[b, :27].astype(np.float64)
-            scale = np.maximum(np.abs(rv).max(axis=1), 1e-30)
-            assert (np.abs(gv - rv).max(axis=1) <= 2e-5 * scale).all()
-            inner = gi[a, 27] >= 0
-            nxt_g.append(gi[a, 27][inner])
-            nxt_r.append(ri[b, 27][inner])
-            nxt_ns.append(gi[a, 30][inner])
-        g, r, ns = (np.concatenate(x).astype(np.int64) for x in (nxt_g, nxt_r, nxt_ns))
-    return nodes
-
-
 def test_c2_headline_full_grid(oracle):
     import torch
     m = W.large_displaced_mesh()
@@ -91,12 +61,12 @@ def test_c2_headline_full_grid(oracle):
     slots, tris, roots, tau = e.export_tree()
     R = oracle.Restated(V, F)
     np.testing.assert_array_equal(tris.reshape(-1, 12)[:, 9], R.tris.reshape(-1, 12)[:, 9])
-    nodes = _compare_trees_levelwise(slots, int(roots[0, 0]), int(roots[0, 1]), R.slots, R.root_ns)
+    nodes = oracle.compare_slot_trees(slots, int(roots[0, 0]), int(roots[0, 1]), R.slots, R.root_ns, 4)
     assert nodes == e.tree_info()["num_nodes"] == R.num_nodes
     # every point: the CPU traversal of the restated tree with the engine's FP64 distance
     om_s, st = e.query_stats(q)
     np.testing.assert_array_equal(om_s, om)  # the counting variant sums the same terms
-    om_r, st_r = oracle.tree_eval(R.slots, 4, 0, R.root_ns, R.tris, q, 2.0, float(tau[0]), stats=True)
+    om_r, st_r = R.solid_angle(q, 2.0, tau=float(tau[0]), stats=True)
     same = (st == st_r).all(axis=1)
     d = np.abs(om.astype(np.float64) - om_r)
     print(f"C2: {same.mean():.5f} of points with identical counters, max |d omega| {d[same].max():.2e} sr there, "
@@ -131,7 +101,7 @@ def test_c3_batched_full_size(oracle):
         wb = single.bruteforce_f64(sub) / FOUR_PI
         R = oracle.Restated(V, F)
         tau = float(single.export_tree()[3][0])
-        wr = oracle.tree_eval(R.slots, 4, 0, R.root_ns, R.tris, sub, 2.0, tau) / FOUR_PI
+        wr = R.solid_angle(sub, 2.0, tau=tau) / FOUR_PI
         d = np.abs(w - wr)  # rare points on a slot's beta boundary flip by one far term
         assert np.mean(d <= 2e-5) >= 0.99 and d.max() <= 1e-3, i
         assert np.abs(w - wb).max() <= np.abs(wr - wb).max() + 1e-4, i

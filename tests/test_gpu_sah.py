@@ -3,7 +3,8 @@ of the reference's 4-ary box-area tree (oracle.Restated), through the C ABI.
 
 With leaf_max 1 the GPU builds the restated tree itself:
   * integer work bit-exact: the sorted triangle order (original facet ids in tree order), and
-    for every node, slot by slot, the covered range, leaf / internal, and the child's slot count;
+    for every node, slot by slot, leaf / internal, a leaf's triangle range, and the child's slot
+    count;
     node count and depth;
   * every slot's float data within 2e-5 of its largest word (the GPU merges moments pairwise
     in FP32 below fp64_moment_min triangles, the restatement merges up to 4 children in FP64);
@@ -24,32 +25,9 @@ pytestmark = pytest.mark.gpu
 SAH = dict(tree_builder=L.FWN_TREE_SAH, leaf_max=1)
 
 
-def _views(slots):
-    s = slots.reshape(-1, 32)
-    return s.view(np.float32), s.view(np.int32)
-
-
 def _compare_trees(gs, groot, gns, rs, rns, width):
-    gf, gi = _views(gs)
-    rf, ri = _views(rs)
-    stack = [(groot, gns, 0, rns)]
-    nodes = 0
-    while stack:
-        gn, gcount, rn, rcount = stack.pop()
-        assert gcount == rcount, (gn, rn)
-        nodes += 1
-        for s in range(gcount):
-            a, b = gn * width + s, rn * width + s
-            assert gi[a, 28] == ri[b, 28] and gi[a, 29] == ri[b, 29], (gn, s)
-            assert (gi[a, 27] < 0) == (ri[b, 27] < 0), (gn, s)
-            assert gi[a, 30] == ri[b, 30], (gn, s)
-            g = gf[a, :27].astype(np.float64)
-            r = rf[b, :27].astype(np.float64)
-            scale = max(np.abs(r).max(), 1e-30)
-            assert np.abs(g - r).max() <= 2e-5 * scale, (gn, s, np.abs(g - r).max() / scale)
-            if gi[a, 27] >= 0:
-                stack.append((gi[a, 27], gi[a, 30], ri[b, 27], ri[b, 30]))
-    return nodes
+    from oracle import oracle as O
+    return O.compare_slot_trees(gs, groot, gns, rs, rns, width)
 
 
 def _meshes():
@@ -99,7 +77,7 @@ def test_sah_queries_match_restated_tree(oracle, open_mesh):
     om, st = e.query_stats(q)
     _, _, _, tau = e.export_tree()
     R = oracle.Restated(V, F)
-    ref, rst = oracle.tree_eval(R.slots, 4, 0, R.root_ns, R.tris, q, 2.0, float(tau[0]), stats=True)
+    ref, rst = R.solid_angle(q, 2.0, tau=float(tau[0]), stats=True)
     same = np.all(st == rst, axis=1)
     assert same.mean() >= 0.995, same.mean()
     d = np.abs(om.astype(np.float64) - ref)
@@ -127,12 +105,11 @@ def test_sah_buckets_and_multi_mesh(oracle):
     V, F = meshes[1]
     e = FastWindingNumber((V, F), tree_builder=L.FWN_TREE_SAH, leaf_max=4)
     slots, tris, roots, tau = e.export_tree()
-    _, it = _views(slots)
-    occ = it[:, 29] > 0
-    assert np.all(it[occ & (it[:, 27] < 0), 29] <= 4)
+    leaf, _, _, _, count, _ = oracle.decode_links(slots)
+    assert np.all(count[leaf] <= 4) and count[leaf].max() > 1
     q = pts[1]
     om1, st1 = e.query_stats(q)
-    ref, rst = oracle.tree_eval(slots, 4, int(roots[0, 0]), int(roots[0, 1]), tris, q, 2.0, float(tau[0]), stats=True)
+    ref, rst = oracle.tree_eval(slots, 4, int(roots[0, 0]), int(roots[0, 1]), tris, q, float(tau[0]), stats=True)
     np.testing.assert_array_equal(st1, rst)
     assert np.abs(om1.astype(np.float64) - ref).max() <= 2e-4
 
@@ -184,13 +161,13 @@ def test_sah_random_meshes_match_restated(oracle, seed):
     beta = float(rng.choice([1.5, 2.0, 3.0]))
     e = FastWindingNumber((V, F), accuracy_scale=beta, **SAH)
     slots, tris, roots, tau = e.export_tree()
-    R = oracle.Restated(V, F)
+    R = oracle.Restated(V, F, beta=beta)  # the threshold words carry beta^2 r^2
     np.testing.assert_array_equal(tris.reshape(-1, 12)[:, 9], R.tris.reshape(-1, 12)[:, 9])
     _compare_trees(slots, int(roots[0, 0]), int(roots[0, 1]), R.slots, R.root_ns, 4)
     lo, hi = V.min(0), V.max(0)
     q = (lo + (hi - lo) * rng.uniform(-0.2, 1.2, (3000, 3))).astype(np.float32)
     om, st = e.query_stats(q)
-    ref, rst = oracle.tree_eval(R.slots, 4, 0, R.root_ns, R.tris, q, beta, float(tau[0]), stats=True)
+    ref, rst = R.solid_angle(q, beta, tau=float(tau[0]), stats=True)
     same = np.all(st == rst, axis=1)
     assert same.mean() >= 0.99, (kind, n, beta, same.mean())
     assert np.abs(om.astype(np.float64) - ref)[same].max() <= 2e-4

@@ -18,6 +18,7 @@ import numpy as np
 import pytest
 
 from lagrange_amd import workloads as W
+from oracle import oracle as oracle_mod
 
 FOUR_PI = 4 * np.pi
 
@@ -118,25 +119,25 @@ def test_moment_identities_closed_mesh(oracle, mesh):
     np.testing.assert_allclose(S, _sym3(X), atol=1e-10 * max(scale, 1e-3))
 
 
-def _pack_slot(rec):
-    """One-slot tree around a moment record (same packing as the engine's slot format)."""
+def _pack_slot(rec, order=2, thr=0.0):
+    """One-slot tree around a moment record (same packing as the engine's slot format), keeping
+    the expansion up to `order`; thr is the far-field threshold word (0: always far)."""
     M = rec[7:16]
     S = rec[16:26]
-    f = np.zeros(32, np.float32)
-    f[0:3] = rec[1:4]
-    f[3] = rec[26]
-    f[4:7] = rec[4:7]
-    f[7] = M[0] + M[4] + M[8]
-    f[8:14] = -3 * np.array([M[0], M[4], M[8], M[1] + M[3], M[5] + M[7], M[6] + M[2]])
-    f[14:24] = np.array([7.5] * 3 + [22.5] * 6 + [45.0]) * S
-    f[24] = -4.5 * (S[0] + S[5] + S[7])
-    f[25] = -4.5 * (S[3] + S[1] + S[8])
-    f[26] = -4.5 * (S[4] + S[6] + S[2])
-    # tr joins the quadratic's diagonal, C the cubic monomials xxx yyy zzz xxy xxz yyx yyz zzx zzy
-    f[8:11] += f[7]
-    f[14:23] += f[[24, 25, 26, 25, 26, 24, 26, 24, 25]]
-    w = f.view(np.uint32).copy()
-    w[27:32] = np.array([-1, 0, 1, 0, 0], np.int32).view(np.uint32)
+    g = np.zeros(23)
+    g[0:3] = rec[1:4]
+    g[3] = thr
+    g[4:7] = rec[4:7]
+    if order >= 1:
+        g[7:13] = -3 * np.array([M[0], M[4], M[8], M[1] + M[3], M[5] + M[7], M[6] + M[2]])
+        g[7:10] += M[0] + M[4] + M[8]  # tr joins the quadratic's diagonal
+    if order >= 2:
+        C = -4.5 * np.array([S[0] + S[5] + S[7], S[3] + S[1] + S[8], S[4] + S[6] + S[2]])
+        # C joins the cubic monomials xxx yyy zzz xxy xxz yyx yyz zzx zzy (xyz has none)
+        g[13:23] = np.array([7.5] * 3 + [22.5] * 6 + [45.0]) * S + np.append(C[[0, 1, 2, 1, 2, 0, 2, 0, 1]], 0.0)
+    w = np.zeros(24, np.uint32)
+    w[:23] = g.astype(np.float32).view(np.uint32)
+    w[23] = oracle_mod.LINK_LEAF  # a leaf of one triangle (never reached: thr 0 keeps it far)
     return w
 
 
@@ -154,18 +155,14 @@ def test_far_field_convergence_order(oracle):
     ks = np.array([8.0, 16.0, 32.0])
     q = (rec[1:4] + d[None, :] * (r * ks)[:, None]).astype(np.float32)
     exact = oracle.bruteforce(V, F[ids], q)
-    # the full expansion, evaluated by the oracle's far-field code (beta tiny -> always far)
-    far2 = oracle.tree_eval(slot, 1, 0, 1, tris, q, beta=1e-3).astype(np.float64)
+    # the full expansion, evaluated by the oracle's far-field code (threshold 0 -> always far)
+    far2 = oracle.tree_eval(slot, 1, 0, 1, tris, q).astype(np.float64)
     e2 = np.abs(far2 - exact)
     slope2 = np.polyfit(np.log(ks), np.log(e2), 1)[0]
     assert slope2 <= -4.5, slope2
     # order 1 only (order-2 words zeroed): slope -4; order 0 only: slope -3
-    s1 = slot.view(np.float32).copy()
-    s1[14:27] = 0
-    e1 = np.abs(oracle.tree_eval(s1.view(np.uint32), 1, 0, 1, tris, q, beta=1e-3) - exact)
-    s0 = s1.copy()
-    s0[7:14] = 0
-    e0 = np.abs(oracle.tree_eval(s0.view(np.uint32), 1, 0, 1, tris, q, beta=1e-3) - exact)
+    e1 = np.abs(oracle.tree_eval(_pack_slot(rec, 1), 1, 0, 1, tris, q) - exact)
+    e0 = np.abs(oracle.tree_eval(_pack_slot(rec, 0), 1, 0, 1, tris, q) - exact)
     assert -4.5 < np.polyfit(np.log(ks), np.log(e1), 1)[0] <= -3.5
     assert -3.5 < np.polyfit(np.log(ks), np.log(e0), 1)[0] <= -2.5
     assert np.all(e2 < e1) and np.all(e1 < e0)
@@ -194,8 +191,8 @@ def test_near_field_fp64_option(oracle, open_mesh):
     R = oracle.Restated(V, F)
     bf = oracle.bruteforce(V, F, q)
     diag = float(np.linalg.norm(V.max(0) - V.min(0)))
-    a = oracle.tree_eval(R.slots, 4, 0, R.root_ns, R.tris, q, 2.0, 0.0)
-    b, st = oracle.tree_eval(R.slots, 4, 0, R.root_ns, R.tris, q, 2.0, 1e-4 * diag, stats=True)
+    a = R.solid_angle(q)
+    b, st = R.solid_angle(q, 2.0, stats=True, tau=1e-4 * diag)
     assert st[:, 3].sum() > 0
     assert np.abs(b - bf).max() / FOUR_PI <= 4e-3 and np.abs(a - bf).max() / FOUR_PI <= 4e-3
 
@@ -206,3 +203,37 @@ def test_lbvh_order_is_sorted_permutation(oracle):
     assert np.array_equal(np.sort(order), np.arange(order.size))
     assert np.all(np.diff(keys.astype(np.float64)) >= 0)
     assert np.all((keys[:ms[0].num_facets] >> 32) == 0) and np.all((keys[ms[0].num_facets:] >> 32) == 1)
+
+
+def _interleave_bits(x, y, z):
+    """30-bit Morton code by the definition, one bit at a time: bit b of x, y, z goes to bits
+    3b + 2, 3b + 1, 3b."""
+    k = 0
+    for b in range(10):
+        k |= ((x >> b) & 1) << (3 * b + 2) | ((y >> b) & 1) << (3 * b + 1) | ((z >> b) & 1) << (3 * b)
+    return k
+
+
+def test_lbvh_morton_keys_known_answers(oracle):
+    """Known answers of the Morton key of lbvh_sorted_order: degenerate triangles whose centroids
+    are exact multiples of 1/1024 in a unit centroid box, so each quantises to its own integer
+    coordinate (1.0 clamps to 1023).  (1,0,0) -> 4, (0,1,0) -> 2, (0,0,1) -> 1, (3,5,6) -> 238 by
+    hand, (1023,1023,1023) -> 2^30 - 1; random coordinates against the bit-by-bit definition; the
+    order is the stable sort of the keys; a second mesh's keys carry mesh id 1 in bits 32+."""
+    rng = np.random.default_rng(19)
+    coords = [(0, 0, 0), (1024, 1024, 1024), (1, 0, 0), (0, 1, 0), (0, 0, 1), (3, 5, 6), (3, 5, 6)]
+    coords += [tuple(int(v) for v in rng.integers(0, 1024, 3)) for _ in range(200)]
+    C = np.array(coords, np.float32) / np.float32(1024)
+    V = np.repeat(C, 3, axis=0)
+    F = np.arange(V.shape[0], dtype=np.int32).reshape(-1, 3)
+    order, keys = oracle.lbvh_sorted_order([V, V], [F, F])
+    n = F.shape[0]
+    q = np.minimum(np.array(coords, np.int64), 1023)
+    want = np.array([_interleave_bits(*c) for c in q], np.uint64)
+    assert want[2] == 4 and want[3] == 2 and want[4] == 1 and want[5] == 238 and want[1] == 2 ** 30 - 1
+    inv = np.empty(2 * n, np.int64)
+    inv[order] = np.arange(2 * n)
+    got = keys[inv]  # key of each input facet
+    np.testing.assert_array_equal(got[:n], want)
+    np.testing.assert_array_equal(got[n:], want | np.uint64(1 << 32))
+    np.testing.assert_array_equal(order[:n], np.argsort(want, kind="stable"))

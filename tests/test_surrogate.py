@@ -189,7 +189,9 @@ def test_c5_surrogate_at_config_size(oracle):
     exact_w = (om.double() / (4 * np.pi)).float()
     assert torch.equal(w[fb], exact_w[fb])
     assert torch.equal(ins[fb], ins_e[fb])
-    idx = torch.linspace(0, n - 1, 512, device=qd.device).long()
+    # sample indices in int64 from a float64 linspace (a float32 one rounds n - 1 = 2^26 - 1 up to n)
+    idx = torch.from_numpy(np.linspace(0, n - 1, 512).astype(np.int64)).to(qd.device)
+    assert int(idx.max()) < n
     wb = oracle.bruteforce(V, F, qd[idx].cpu().numpy()) / (4 * np.pi)
     band = np.abs(wb - 0.5) > 1e-3
     np.testing.assert_array_equal(ins[idx].cpu().numpy()[band], (wb > 0.5)[band])
