@@ -301,12 +301,14 @@ template <>
 struct WaveStack<1> {
     static constexpr int kCap = 64;
     uint32_t n0 = 0, l0 = 0, h0 = 0;
+    // sp < kCap is the caller's contract (the host launches this stack only for trees whose
+    // bound fits; callers still flag sp >= kCap as an overflow): no guard here, so the push is
+    // three writelanes with no branch around them
     __device__ __forceinline__ void push(int sp, int /*lane*/, uint32_t e, uint64_t mk)
     {
-        if (sp >= kCap) return;
-        n0 = wave_writelane(e, sp, n0);
-        l0 = wave_writelane((uint32_t)mk, sp, l0);
-        h0 = wave_writelane((uint32_t)(mk >> 32), sp, h0);
+        n0 = wave_writelane(e, sp & 63, n0);
+        l0 = wave_writelane((uint32_t)mk, sp & 63, l0);
+        h0 = wave_writelane((uint32_t)(mk >> 32), sp & 63, h0);
     }
     __device__ __forceinline__ void pop(int sp, uint32_t& e, uint64_t& mk) const
     {
@@ -355,11 +357,13 @@ struct WaveStack<3> {
     }
 };
 
-// stack banks a tree of this width and depth needs (1 or 3), 0 if it is too deep
+// stack banks a tree of this width and depth needs (1 or 3), 0 if it is too deep.  The bound
+// 1 + (width - 1) depth on the live entries keeps one position free above the top: the traversal
+// writes that position for leaf slots too (and only advances over it for internal ones).
 inline int stack_banks(int width, int depth)
 {
     const int need = 1 + (width - 1) * depth;
-    return need <= 64 ? 1 : (need <= kStack ? 3 : 0);
+    return need < 64 ? 1 : (need < kStack ? 3 : 0);
 }
 
 // XCD-aware block remap: hardware hands consecutive workgroups to the 8 XCDs round-robin;

@@ -259,8 +259,17 @@ __device__ __forceinline__ void traverse_batch(
                     if (STATS) ++n_far;
                 }
                 const uint32_t link = su(cur, FWN_W_LINK);
-                const bool leaf = (link & FWN_LINK_LEAF) != 0u;
-                if (nearm != 0ull && leaf) {
+                if (nearm == 0ull) return;
+                // written above the stack top for every slot with near lanes, kept (sp advanced)
+                // only for an internal slot: the writelanes then sit in a branch-free block and
+                // the stack registers need no copies.  The position written is below the
+                // capacity (the host keeps a one-entry margin for it), so the entries below stay
+                // intact; an overflow is checked once per node.  The link of an internal slot is
+                // the stack entry: (child << 4) | its slot count
+                S.push(sp, lane, link, nearm);
+                const bool leaf = (int32_t)link < 0; // FWN_LINK_LEAF is the sign bit
+                sp += leaf ? 0 : 1;
+                if (leaf) {
                     const int32_t first = (int32_t)((link & ~FWN_LINK_LEAF) >> 3);
                     const int32_t tcnt = (int32_t)(link & 7u) + 1;
                     const bool nl = __builtin_amdgcn_inverse_ballot_w64(nearm);
@@ -313,16 +322,12 @@ __device__ __forceinline__ void traverse_batch(
                         ch.fill += c;
                     }
                 }
-                if (nearm != 0ull && !leaf) {
-                    // the host only launches trees whose stack bound 1 + (width - 1) depth fits;
-                    // past the capacity a push writes no lane and the overflow is reported.  The
-                    // link of an internal slot is the stack entry: (child << 4) | its slot count
-                    overflow |= sp >= WaveStack<NB>::kCap;
-                    S.push(sp, lane, link, nearm);
-                    ++sp;
-                }
             };
-            for (int s = 0; s < ns; ++s) slot(load_slot(nb + s * kSlotWords));
+            const uint32_t* sl = nb;
+            for (int s = 0; s < ns; ++s, sl += kSlotWords) slot(load_slot(sl));
+            // the host only launches trees whose stack bound 1 + (width - 1) depth fits; a push
+            // past the capacity is reported
+            overflow |= sp > WaveStack<NB>::kCap;
         }
         if (overflow && lane == 0) atomicOr(flag, 1);
     }
@@ -433,22 +438,18 @@ __global__ __launch_bounds__(256) void k_bruteforce(const uint32_t* __restrict__
 }
 
 // ---------------------------------------------------------------------------------------
-// Single-point calls (the reference's per-point solid_angle / is_inside): one wave per point,
-// the lanes of the wave taking the slots of a node in parallel.  A point walks the same tree in
-// the same order as in the batched traversal (children pushed in slot order, popped last-in-
-// first-out), so the walk is a chain of dependent node loads; here each node costs one load
-// round trip (lane s loads slot s) and one parallel test / far-field evaluation, instead of one
-// scalar load and one far-field evaluation per slot in turn.  The terms are then added in the
-// batched traversal's order (slot order, the same FP32 operations: far terms as fmaf(t, inv3,
-// omega), near FP32 triangles one by one) and the FP64 near leaves in 2^-40 fixed point, so a
-// point gets bit for bit the value a batched call gives it.  Points and the launch come in the
-// kernel arguments; results are written straight to pinned host memory.
-constexpr int kPointArgs = 16; // points per single-point launch
-
-struct PointArgs {
-    float q[kPointArgs][3];
-};
-
+// Single-point calls (the reference's per-point solid_angle / is_inside) and other host calls of
+// a few points: one wave per point, the lanes of the wave taking the slots of a node in
+// parallel.  A point walks the same tree in the same order as in the batched traversal
+// (children pushed in slot order, popped last-in-first-out), so the walk is a chain of dependent
+// node loads.  Here each node is tested and expanded in one parallel step (lane s takes slot s),
+// and with PF every pushed child's node is fetched at push time straight into the LDS buffer of
+// its stack position (global_load_lds): the child popped next waits for one load round trip, its
+// siblings are already there when their turn comes.  The terms are then added in the batched
+// traversal's order (slot order, the same FP32 operations: far terms as fmaf(t, inv3, omega),
+// near FP32 triangles one by one) and the FP64 near leaves in 2^-40 fixed point, so a point gets
+// bit for bit the value a batched call gives it.  Points are read from and results written to
+// pinned host memory.
 struct LaneWords {
     const float* w;
     __device__ __forceinline__ float operator()(int k) const { return w[k]; }
@@ -459,20 +460,30 @@ __device__ __forceinline__ float readlane_f(float v, int l)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
-template <int NB>
+// one node (width x 96 bytes) into LDS: lane l < width * 6 copies 16 bytes
+__device__ __forceinline__ void fetch_node(const uint32_t* __restrict__ nb, uint4* lds, int width, int lane)
+{
+    if (lane < width * (kSlotWords / 4))
+        __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint4*>(nb) + lane, lds, 16, 0, 0);
+}
+
+template <int NB, bool PF>
 __global__ __launch_bounds__(64) void k_point(const uint32_t* __restrict__ slots, int width,
                                               const uint32_t* __restrict__ tris, int32_t root, int32_t root_ns,
-                                              float tau, PointArgs pa, float* __restrict__ omega,
-                                              uint8_t* __restrict__ inside, int32_t* __restrict__ flag)
+                                              float tau, const float* __restrict__ Q, float* __restrict__ omega,
+                                              uint8_t* __restrict__ inside, int32_t* __restrict__ flag, int lds_nodes)
 {
+    extern __shared__ uint4 nodebuf[]; // PF: one node per stack position (width * 6 uint4 each), lds_nodes of them
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
-    const float qx = pa.q[i][0], qy = pa.q[i][1], qz = pa.q[i][2];
+    const int nwords = width * (kSlotWords / 4); // uint4 per node
+    const float qx = Q[3 * (size_t)i], qy = Q[3 * (size_t)i + 1], qz = Q[3 * (size_t)i + 2];
     float om = 0.0f;
     unsigned long long acc = 0ull; // FP64 near leaves, 2^-40 fixed point
     if (root >= 0) {
         WaveStack<NB> S;
         S.push(0, lane, ((uint32_t)root << 4) | (uint32_t)root_ns, 0ull);
+        if (PF) fetch_node(slots + (size_t)root * (size_t)width * kSlotWords, nodebuf, width, lane);
         int sp = 1;
         bool overflow = false;
         while (sp > 0) {
@@ -482,14 +493,16 @@ __global__ __launch_bounds__(64) void k_point(const uint32_t* __restrict__ slots
             S.pop(sp, e, unused);
             const int ns = (int)(e & 15u);
             if (ns == 0) continue;
-            const uint32_t* nb = slots + (size_t)(e >> 4) * (size_t)width * kSlotWords;
-            // parallel phase: lane s < ns loads, tests and (when far) expands slot s
+            // parallel phase: lane s < ns reads, tests and (when far) expands slot s
             float w[FWN_W_LINK];
             uint32_t link = FWN_LINK_EMPTY;
             float dx = 0.0f, dy = 0.0f, dz = 0.0f, d2 = 0.0f, t = 0.0f, inv3 = 0.0f;
             bool far = false;
-            if (lane < ns) {
-                const uint4* src = reinterpret_cast<const uint4*>(nb + (size_t)lane * kSlotWords);
+            if (PF) asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); // this node's fetch has landed
+            if (PF && sp >= lds_nodes) overflow = true; // never with the host's stack bound
+            if (lane < ns && !(PF && sp >= lds_nodes)) {
+                const uint4* src = PF ? nodebuf + (size_t)sp * nwords + lane * (kSlotWords / 4)
+                                      : reinterpret_cast<const uint4*>(slots + ((size_t)(e >> 4) * (size_t)width + lane) * kSlotWords);
                 uint32_t u[kSlotWords];
 #pragma unroll
                 for (int k = 0; k < kSlotWords / 4; ++k) {
@@ -512,7 +525,8 @@ __global__ __launch_bounds__(64) void k_point(const uint32_t* __restrict__ slots
                 far_field_terms(LaneWords{w}, dx, dy, dz, d2, t, inv3);
             }
             const uint64_t farm = __ballot(far);
-            // near leaves: the FP64 rule's thresholds from the leaf's first triangle
+            // near leaves: the FP64 rule's thresholds from the leaf's first triangle (loaded before the
+            // children's fetches, so that its wait does not include them)
             const bool nleaf = lane < ns && !far && (link & FWN_LINK_LEAF) != 0u;
             bool use64 = false;
             if (nleaf && tau > 0.0f) {
@@ -524,12 +538,24 @@ __global__ __launch_bounds__(64) void k_point(const uint32_t* __restrict__ slots
                 use64 = (d2 < np2.x) & ((tcnt > 1) | (nd * nd < np2.y)); // no short circuit: one load
             }
             const uint64_t m64 = __ballot(use64);
-            // serial phase: the terms in slot order, children pushed in slot order
+            // children first, in slot order, each fetched as it is pushed (before the serial sums)
+            const uint64_t pushm = __ballot(lane < ns && !far && (link & FWN_LINK_LEAF) == 0u);
             for (int s2 = 0; s2 < ns; ++s2) {
+                if (!((pushm >> s2) & 1ull)) continue;
                 const uint32_t ls = (uint32_t)__builtin_amdgcn_readlane((int)link, s2);
+                overflow |= sp >= WaveStack<NB>::kCap;
+                S.push(sp, lane, ls, 0ull);
+                if (PF && sp < lds_nodes)
+                    fetch_node(slots + (size_t)(ls >> 4) * (size_t)width * kSlotWords, nodebuf + (size_t)sp * nwords,
+                               width, lane);
+                ++sp;
+            }
+            // serial phase: the terms in slot order
+            for (int s2 = 0; s2 < ns; ++s2) {
                 if ((farm >> s2) & 1ull) {
                     om = fmaf(readlane_f(t, s2), readlane_f(inv3, s2), om);
-                } else if (ls & FWN_LINK_LEAF) {
+                } else if (!((pushm >> s2) & 1ull)) {
+                    const uint32_t ls = (uint32_t)__builtin_amdgcn_readlane((int)link, s2);
                     const int32_t first = (int32_t)((ls & ~FWN_LINK_LEAF) >> 3);
                     const int32_t tcnt = (int32_t)(ls & 7u) + 1;
                     if ((m64 >> s2) & 1ull) {
@@ -541,10 +567,6 @@ __global__ __launch_bounds__(64) void k_point(const uint32_t* __restrict__ slots
                         for (int32_t k = first; k < first + tcnt; ++k)
                             om += tri_f32(reinterpret_cast<const float*>(tris + (size_t)k * kTriWords), qx, qy, qz);
                     }
-                } else {
-                    overflow |= sp >= WaveStack<NB>::kCap;
-                    S.push(sp, lane, ls, 0ull);
-                    ++sp;
                 }
             }
         }
@@ -822,29 +844,39 @@ void run_queries_pipelined(Engine& e, const float* Q, int64_t nQ, float* omega, 
     }
 }
 
-// Host calls of at most kPointArgs points on a single-mesh engine: one k_point launch (points
-// in the arguments, results into the pinned stage) and one wait.  Per-point calls record no
-// device time.
+// Host calls of at most kSmallBatch points on a single-mesh engine: the points go to the pinned
+// stage, one k_point launch reads them there and writes the results back into it, one wait.
+// The LDS node buffers (one per stack position the tree can need) are used when they fit in
+// kPointLds bytes.  These calls record no device time.
+constexpr size_t kPointLds = 32768;
+
 void run_points(Engine& e, const float* Q, int64_t nQ, float* omega, uint8_t* inside, int banks)
 {
     hipStream_t s = e.stream;
-    const size_t out_b = 16 + (sizeof(float) + 1) * (size_t)nQ; // flag, omega, inside
+    const size_t pts_b = (sizeof(float) * 3 * (size_t)nQ + 15) & ~(size_t)15;
+    const size_t out_b = 16 + pts_b + (sizeof(float) + 1) * (size_t)nQ; // flag, points, omega, inside
     char* hs = host_stage(e, out_b);
     int32_t* hflag = reinterpret_cast<int32_t*>(hs);
-    float* hom = reinterpret_cast<float*>(hs + 16);
-    uint8_t* hin = reinterpret_cast<uint8_t*>(hs + 16 + sizeof(float) * (size_t)nQ);
+    float* hq = reinterpret_cast<float*>(hs + 16);
+    float* hom = reinterpret_cast<float*>(hs + 16 + pts_b);
+    uint8_t* hin = reinterpret_cast<uint8_t*>(hs + 16 + pts_b + sizeof(float) * (size_t)nQ);
     *hflag = 0;
-    PointArgs pa;
-    std::memset(&pa, 0, sizeof(pa));
-    std::memcpy(pa.q, Q, sizeof(float) * 3 * (size_t)nQ);
+    std::memcpy(hq, Q, sizeof(float) * 3 * (size_t)nQ);
     const int32_t root = e.h_roots[0], root_ns = e.h_roots[1];
     const float tau = e.h_tau[0];
-    if (banks == 1)
-        hipLaunchKernelGGL(k_point<1>, dim3((unsigned)nQ), dim3(64), 0, s, e.slots.as<uint32_t>(), e.width,
-                           e.tris.as<uint32_t>(), root, root_ns, tau, pa, hom, hin, hflag);
+    const size_t need = (size_t)(1 + (e.width - 1) * std::max(e.max_depth, 1)); // stack entries
+    const size_t lds = need * (size_t)e.width * kSlotWords * sizeof(uint32_t);
+    const bool pf = banks == 1 && lds <= kPointLds;
+    const dim3 grid((unsigned)nQ), block(64);
+    if (pf)
+        hipLaunchKernelGGL((k_point<1, true>), grid, block, lds, s, e.slots.as<uint32_t>(), e.width,
+                           e.tris.as<uint32_t>(), root, root_ns, tau, hq, hom, hin, hflag, (int)need);
+    else if (banks == 1)
+        hipLaunchKernelGGL((k_point<1, false>), grid, block, 0, s, e.slots.as<uint32_t>(), e.width,
+                           e.tris.as<uint32_t>(), root, root_ns, tau, hq, hom, hin, hflag, 0);
     else
-        hipLaunchKernelGGL(k_point<3>, dim3((unsigned)nQ), dim3(64), 0, s, e.slots.as<uint32_t>(), e.width,
-                           e.tris.as<uint32_t>(), root, root_ns, tau, pa, hom, hin, hflag);
+        hipLaunchKernelGGL((k_point<3, false>), grid, block, 0, s, e.slots.as<uint32_t>(), e.width,
+                           e.tris.as<uint32_t>(), root, root_ns, tau, hq, hom, hin, hflag, 0);
     FWN_LAUNCH_CHECK("k_point");
     FWN_HIP(hipStreamSynchronize(s));
     if (*hflag) fail(FWN_LIMIT_EXCEEDED, "traversal stack overflow");
@@ -909,7 +941,7 @@ void run_queries(Engine& e, const float* Q, const int64_t* q_off_host, int64_t n
         run_queries_pipelined(e, Q, nQ, omega, inside, banks);
         return;
     }
-    if (!dev && !stats && M == 1 && nQ <= kPointArgs) {
+    if (!dev && !stats && M == 1 && nQ <= kSmallBatch) {
         run_points(e, Q, nQ, omega, inside, banks);
         return;
     }

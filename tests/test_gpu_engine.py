@@ -432,10 +432,12 @@ def _telescope_soup(n=100):
 
 @pytest.mark.parametrize("case", ["ico5_fp64", "open_leaf4", "width2", "width8_lbvh", "deep"])
 def test_point_kernel_equals_batched(ico5, open_mesh, case):
-    """Calls of at most 16 host points run the single-point kernel (one wave per point, the
-    node's slots in parallel lanes, terms added in the batched traversal's order): every point
-    equals the batched call bit for bit, Omega and flag, with FP64 near leaves, leaf buckets,
-    tree widths 2 / 8, the Morton builder and a tree deep enough for the three-bank stack."""
+    """Host calls of at most 2,048 points on a single-mesh engine run the single-point kernel (one
+    wave per point, the node's slots in parallel lanes, children fetched into LDS as they are
+    pushed, terms added in the batched traversal's order): every point equals the batched call
+    bit for bit, Omega and flag, in calls of 1 to 16 points and of 300 and 2,048 points, with FP64
+    near leaves, leaf buckets, tree widths 2 / 8, the Morton builder and a tree deep enough for the
+    three-bank stack (which reads its nodes without the LDS buffers)."""
     opts = {}
     if case == "ico5_fp64":
         V, F = ico5
@@ -477,6 +479,13 @@ def test_point_kernel_equals_batched(ico5, open_mesh, case):
     for k in range(0, q.shape[0], 37):
         assert e.solid_angle(q[k]) == om[k]
         assert e.is_inside(q[k]) == bool(ins[k])
+    big = np.concatenate([q] * (2048 // q.shape[0] + 1))[:2048]
+    om_big, in_big = e.solid_angle_batch(big, with_inside=True)
+    np.testing.assert_array_equal(om_big, np.concatenate([om] * (2048 // q.shape[0] + 1))[:2048])
+    np.testing.assert_array_equal(in_big, np.concatenate([ins] * (2048 // q.shape[0] + 1))[:2048])
+    a, b = e.solid_angle_batch(q[:300], with_inside=True)
+    np.testing.assert_array_equal(a, om[:300])
+    np.testing.assert_array_equal(b, ins[:300])
 
 
 def test_repeated_call_shapes_reuse_metadata(ico5):
