@@ -184,6 +184,20 @@ int fwn_destroy(fwn_engine* e);
 int fwn_solid_angle(fwn_engine* e, const float* Q, int64_t nQ, float* omega_out, uint8_t* inside_out,
                     int32_t flags);
 
+/* No-allocation variant of fwn_solid_angle (no counterpart in the reference), for callers that
+ * cannot allocate device memory during a call: code in HIP graph capture regions, tight serving
+ * loops, harnesses that forbid hipMalloc.  Same results as fwn_solid_angle with
+ * FWN_DEVICE_POINTERS, bit for bit, for a single-mesh engine and at most FWN_NOALLOC_MAX_POINTS
+ * points (more: FWN_LIMIT_EXCEEDED).  Q (nQ x 3 float), omega_out and inside_out (nullable) are
+ * device pointers.  The call allocates nothing (no hipMalloc / hipFree, no pinned host memory):
+ * its scratch (point keys and order, sort histograms, call metadata and flags, about 17 MB) is one
+ * fixed device array of the library, which every GPU the library runs on holds until the process
+ * ends.  Calls on one GPU share that array and would race if they overlapped: the library
+ * serialises them with a process-wide lock, and each call returns only after its kernels have
+ * finished. */
+#define FWN_NOALLOC_MAX_POINTS (1 << 20)
+int fwn_solid_angle_noalloc(fwn_engine* e, const float* Q, int64_t nQ, float* omega_out, uint8_t* inside_out);
+
 /* Points of every mesh in one launch: points of mesh m are [q_off[m], q_off[m+1]) of Q. */
 int fwn_solid_angle_multi(fwn_engine* e, const float* Q, const int64_t* q_off, float* omega_out,
                           uint8_t* inside_out, int32_t flags);

@@ -104,6 +104,7 @@ def load():
         "fwn_create_multi": ([C.POINTER(VP), I32, VP, VP, VP, VP, C.POINTER(Options)], I32),
         "fwn_destroy": ([VP], I32),
         "fwn_solid_angle": ([VP, VP, I64, VP, VP, I32], I32),
+        "fwn_solid_angle_noalloc": ([VP, VP, I64, VP, VP], I32),
         "fwn_solid_angle_multi": ([VP, VP, VP, VP, VP, I32], I32),
         "fwn_solid_angle_point": ([VP, VP, VP], I32),
         "fwn_is_inside_point": ([VP, VP, VP], I32),

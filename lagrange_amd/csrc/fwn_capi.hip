@@ -286,6 +286,17 @@ int fwn_solid_angle(fwn_engine* h, const float* Q, int64_t nQ, float* omega_out,
     });
 }
 
+int fwn_solid_angle_noalloc(fwn_engine* h, const float* Q, int64_t nQ, float* omega_out, uint8_t* inside_out)
+{
+    return guarded("fwn_solid_angle_noalloc", [&] {
+        fwn::Engine* e = E(h);
+        if (nQ > 0 && (!Q || !omega_out)) fwn::fail(FWN_INVALID_ARGUMENT, "NULL points or output");
+        std::lock_guard<std::mutex> lk(e->mu);
+        fwn::DeviceGuard g(e->device);
+        fwn::run_queries_noalloc(*e, Q, nQ, omega_out, inside_out);
+    });
+}
+
 int fwn_solid_angle_multi(fwn_engine* h, const float* Q, const int64_t* q_off, float* omega_out, uint8_t* inside_out,
                           int32_t flags)
 {

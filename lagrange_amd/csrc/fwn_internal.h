@@ -173,6 +173,9 @@ void run_bruteforce(Engine& e, const float* Q, int64_t nQ, double* out, bool dev
 void launch_point_keys(Engine& e, const float* dQ, int64_t nQ, uint32_t* keys, hipStream_t s);
 void launch_traverse_inplace(Engine& e, int banks, const float* dQ, const uint32_t* perm, const int64_t* meta,
                              int64_t nQ, float* omega, uint8_t* inside, int32_t* flag, hipStream_t s);
+// fwn_solid_angle_noalloc (fwn_noalloc.hip): device points of a single-mesh engine through a fixed
+// device scratch array; no allocation
+void run_queries_noalloc(Engine& e, const float* dQ, int64_t nQ, float* dOm, uint8_t* dIn);
 // Morton order of nQ device points of a single-mesh engine: returns the permutation (sorted
 // position -> original index) in the engine's query scratch, valid until the next query call.
 const uint32_t* sort_points_single(Engine& e, const float* dQ, int64_t nQ);
