@@ -460,6 +460,13 @@ __device__ __forceinline__ float readlane_f(float v, int l)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
+// calls of up to kPointArgs points pass them in the kernel arguments (no read of host memory
+// across the bus before the first node)
+constexpr int kPointArgs = 16;
+struct PointArgs {
+    float q[kPointArgs][3];
+};
+
 // one node (width x 96 bytes) into LDS: lane l < width * 6 copies 16 bytes
 __device__ __forceinline__ void fetch_node(const uint32_t* __restrict__ nb, uint4* lds, int width, int lane)
 {
@@ -470,14 +477,17 @@ __device__ __forceinline__ void fetch_node(const uint32_t* __restrict__ nb, uint
 template <int NB, bool PF>
 __global__ __launch_bounds__(64) void k_point(const uint32_t* __restrict__ slots, int width,
                                               const uint32_t* __restrict__ tris, int32_t root, int32_t root_ns,
-                                              float tau, const float* __restrict__ Q, float* __restrict__ omega,
-                                              uint8_t* __restrict__ inside, int32_t* __restrict__ flag, int lds_nodes)
+                                              float tau, const float* __restrict__ Q, PointArgs pa,
+                                              float* __restrict__ omega, uint8_t* __restrict__ inside,
+                                              int32_t* __restrict__ flag, int lds_nodes)
 {
     extern __shared__ uint4 nodebuf[]; // PF: one node per stack position (width * 6 uint4 each), lds_nodes of them
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     const int nwords = width * (kSlotWords / 4); // uint4 per node
-    const float qx = Q[3 * (size_t)i], qy = Q[3 * (size_t)i + 1], qz = Q[3 * (size_t)i + 2];
+    const float qx = Q ? Q[3 * (size_t)i] : pa.q[i][0];
+    const float qy = Q ? Q[3 * (size_t)i + 1] : pa.q[i][1];
+    const float qz = Q ? Q[3 * (size_t)i + 2] : pa.q[i][2];
     float om = 0.0f;
     unsigned long long acc = 0ull; // FP64 near leaves, 2^-40 fixed point
     if (root >= 0) {
@@ -844,8 +854,9 @@ void run_queries_pipelined(Engine& e, const float* Q, int64_t nQ, float* omega, 
     }
 }
 
-// Host calls of at most kSmallBatch points on a single-mesh engine: the points go to the pinned
-// stage, one k_point launch reads them there and writes the results back into it, one wait.
+// Host calls of at most kSmallBatch points on a single-mesh engine: one k_point launch (points in
+// its arguments up to kPointArgs, else read from the pinned stage) writes the results into the
+// pinned stage, one wait.
 // The LDS node buffers (one per stack position the tree can need) are used when they fit in
 // kPointLds bytes.  These calls record no device time.
 constexpr size_t kPointLds = 32768;
@@ -861,7 +872,15 @@ void run_points(Engine& e, const float* Q, int64_t nQ, float* omega, uint8_t* in
     float* hom = reinterpret_cast<float*>(hs + 16 + pts_b);
     uint8_t* hin = reinterpret_cast<uint8_t*>(hs + 16 + pts_b + sizeof(float) * (size_t)nQ);
     *hflag = 0;
-    std::memcpy(hq, Q, sizeof(float) * 3 * (size_t)nQ);
+    PointArgs pa;
+    std::memset(&pa, 0, sizeof(pa));
+    const float* dq = hq; // points read from the pinned stage, or from the arguments (nullptr)
+    if (nQ <= kPointArgs) {
+        std::memcpy(pa.q, Q, sizeof(float) * 3 * (size_t)nQ);
+        dq = nullptr;
+    } else {
+        std::memcpy(hq, Q, sizeof(float) * 3 * (size_t)nQ);
+    }
     const int32_t root = e.h_roots[0], root_ns = e.h_roots[1];
     const float tau = e.h_tau[0];
     const size_t need = (size_t)(1 + (e.width - 1) * std::max(e.max_depth, 1)); // stack entries
@@ -870,13 +889,13 @@ void run_points(Engine& e, const float* Q, int64_t nQ, float* omega, uint8_t* in
     const dim3 grid((unsigned)nQ), block(64);
     if (pf)
         hipLaunchKernelGGL((k_point<1, true>), grid, block, lds, s, e.slots.as<uint32_t>(), e.width,
-                           e.tris.as<uint32_t>(), root, root_ns, tau, hq, hom, hin, hflag, (int)need);
+                           e.tris.as<uint32_t>(), root, root_ns, tau, dq, pa, hom, hin, hflag, (int)need);
     else if (banks == 1)
         hipLaunchKernelGGL((k_point<1, false>), grid, block, 0, s, e.slots.as<uint32_t>(), e.width,
-                           e.tris.as<uint32_t>(), root, root_ns, tau, hq, hom, hin, hflag, 0);
+                           e.tris.as<uint32_t>(), root, root_ns, tau, dq, pa, hom, hin, hflag, 0);
     else
         hipLaunchKernelGGL((k_point<3, false>), grid, block, 0, s, e.slots.as<uint32_t>(), e.width,
-                           e.tris.as<uint32_t>(), root, root_ns, tau, hq, hom, hin, hflag, 0);
+                           e.tris.as<uint32_t>(), root, root_ns, tau, dq, pa, hom, hin, hflag, 0);
     FWN_LAUNCH_CHECK("k_point");
     FWN_HIP(hipStreamSynchronize(s));
     if (*hflag) fail(FWN_LIMIT_EXCEEDED, "traversal stack overflow");
