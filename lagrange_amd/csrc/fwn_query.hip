@@ -19,6 +19,7 @@
 // use the FP64 van Oosterom-Strackee formula.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstring>
 #include <thread>
 
@@ -479,7 +480,7 @@ __global__ __launch_bounds__(64) void k_point(const uint32_t* __restrict__ slots
                                               const uint32_t* __restrict__ tris, int32_t root, int32_t root_ns,
                                               float tau, const float* __restrict__ Q, PointArgs pa,
                                               float* __restrict__ omega, uint8_t* __restrict__ inside,
-                                              int32_t* __restrict__ flag, int lds_nodes)
+                                              int32_t* __restrict__ flag, int lds_nodes, uint8_t* __restrict__ done)
 {
     extern __shared__ uint4 nodebuf[]; // PF: one node per stack position (width * 6 uint4 each), lds_nodes of them
     const int i = blockIdx.x;
@@ -587,6 +588,10 @@ __global__ __launch_bounds__(64) void k_point(const uint32_t* __restrict__ slots
     if (lane == 0) {
         omega[i] = om;
         if (inside) inside[i] = ((double)om / (4.0 * 3.14159265358979323846) > 0.5) ? 1 : 0;
+        if (done) { // the host waits on this byte: the results and the overflow flag go first
+            __threadfence_system();
+            *(volatile uint8_t*)(done + i) = 1;
+        }
     }
 }
 
@@ -865,13 +870,16 @@ void run_points(Engine& e, const float* Q, int64_t nQ, float* omega, uint8_t* in
 {
     hipStream_t s = e.stream;
     const size_t pts_b = (sizeof(float) * 3 * (size_t)nQ + 15) & ~(size_t)15;
-    const size_t out_b = 16 + pts_b + (sizeof(float) + 1) * (size_t)nQ; // flag, points, omega, inside
+    // stage: overflow flag (16 B), done bytes of argument calls (16 B), points, omega, inside
+    const size_t out_b = 32 + pts_b + (sizeof(float) + 1) * (size_t)nQ;
     char* hs = host_stage(e, out_b);
     int32_t* hflag = reinterpret_cast<int32_t*>(hs);
-    float* hq = reinterpret_cast<float*>(hs + 16);
-    float* hom = reinterpret_cast<float*>(hs + 16 + pts_b);
-    uint8_t* hin = reinterpret_cast<uint8_t*>(hs + 16 + pts_b + sizeof(float) * (size_t)nQ);
+    uint8_t* hdone = reinterpret_cast<uint8_t*>(hs + 16);
+    float* hq = reinterpret_cast<float*>(hs + 32);
+    float* hom = reinterpret_cast<float*>(hs + 32 + pts_b);
+    uint8_t* hin = reinterpret_cast<uint8_t*>(hs + 32 + pts_b + sizeof(float) * (size_t)nQ);
     *hflag = 0;
+    std::memset(hdone, 0, 16);
     PointArgs pa;
     std::memset(&pa, 0, sizeof(pa));
     const float* dq = hq; // points read from the pinned stage, or from the arguments (nullptr)
@@ -887,17 +895,35 @@ void run_points(Engine& e, const float* Q, int64_t nQ, float* omega, uint8_t* in
     const size_t lds = need * (size_t)e.width * kSlotWords * sizeof(uint32_t);
     const bool pf = banks == 1 && lds <= kPointLds;
     const dim3 grid((unsigned)nQ), block(64);
+    // argument calls (the per-point API) wait by polling the done bytes the kernel writes after
+    // its results: no stream synchronisation in the round trip.  Every 256 polls the stream is
+    // asked whether it has finished, so a failed launch ends the wait.
+    uint8_t* done = dq ? nullptr : hdone;
     if (pf)
         hipLaunchKernelGGL((k_point<1, true>), grid, block, lds, s, e.slots.as<uint32_t>(), e.width,
-                           e.tris.as<uint32_t>(), root, root_ns, tau, dq, pa, hom, hin, hflag, (int)need);
+                           e.tris.as<uint32_t>(), root, root_ns, tau, dq, pa, hom, hin, hflag, (int)need, done);
     else if (banks == 1)
         hipLaunchKernelGGL((k_point<1, false>), grid, block, 0, s, e.slots.as<uint32_t>(), e.width,
-                           e.tris.as<uint32_t>(), root, root_ns, tau, dq, pa, hom, hin, hflag, 0);
+                           e.tris.as<uint32_t>(), root, root_ns, tau, dq, pa, hom, hin, hflag, 0, done);
     else
         hipLaunchKernelGGL((k_point<3, false>), grid, block, 0, s, e.slots.as<uint32_t>(), e.width,
-                           e.tris.as<uint32_t>(), root, root_ns, tau, dq, pa, hom, hin, hflag, 0);
+                           e.tris.as<uint32_t>(), root, root_ns, tau, dq, pa, hom, hin, hflag, 0, done);
     FWN_LAUNCH_CHECK("k_point");
-    FWN_HIP(hipStreamSynchronize(s));
+    bool polled = false;
+    if (done) {
+        const volatile uint8_t* vd = hdone;
+        for (uint32_t spin = 1;; ++spin) {
+            int64_t k = 0;
+            while (k < nQ && vd[k] != 0) ++k;
+            if (k == nQ) {
+                polled = true;
+                break;
+            }
+            if ((spin & 255u) == 0u && hipStreamQuery(s) != hipErrorNotReady) break;
+        }
+        std::atomic_thread_fence(std::memory_order_acquire);
+    }
+    if (!polled) FWN_HIP(hipStreamSynchronize(s));
     if (*hflag) fail(FWN_LIMIT_EXCEEDED, "traversal stack overflow");
     std::memcpy(omega, hom, sizeof(float) * (size_t)nQ);
     if (inside) std::memcpy(inside, hin, (size_t)nQ);
