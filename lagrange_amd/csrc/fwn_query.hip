@@ -440,17 +440,18 @@ __global__ __launch_bounds__(256) void k_bruteforce(const uint32_t* __restrict__
 
 // ---------------------------------------------------------------------------------------
 // Single-point calls (the reference's per-point solid_angle / is_inside) and other host calls of
-// a few points: one wave per point, the lanes of the wave taking the slots of a node in
-// parallel.  A point walks the same tree in the same order as in the batched traversal
-// (children pushed in slot order, popped last-in-first-out), so the walk is a chain of dependent
-// node loads.  Here each node is tested and expanded in one parallel step (lane s takes slot s),
-// and with PF every pushed child's node is fetched at push time straight into the LDS buffer of
-// its stack position (global_load_lds): the child popped next waits for one load round trip, its
-// siblings are already there when their turn comes.  The terms are then added in the batched
-// traversal's order (slot order, the same FP32 operations: far terms as fmaf(t, inv3, omega),
-// near FP32 triangles one by one) and the FP64 near leaves in 2^-40 fixed point, so a point gets
-// bit for bit the value a batched call gives it.  Points are read from and results written to
-// pinned host memory.
+// a few points run k_point_bfs (below); k_point answers the points whose breadth-first walk does
+// not fit in its LDS records.  k_point: one wave per point, the lanes of the wave taking the
+// slots of a node in parallel.  A point walks the same tree in the same order as in the batched
+// traversal (children pushed in slot order, popped last-in-first-out), so the walk is a chain of
+// dependent node loads.  Here each node is tested and expanded in one parallel step (lane s
+// takes slot s), and with PF every pushed child's node is fetched at push time straight into the
+// LDS buffer of its stack position (global_load_lds): the child popped next waits for one load
+// round trip, its siblings are already there when their turn comes.  The terms are then added
+// in the batched traversal's order (slot order, the same FP32 operations: far terms as
+// fmaf(t, inv3, omega), near FP32 triangles one by one) and the FP64 near leaves in 2^-40 fixed
+// point, so a point gets bit for bit the value a batched call gives it.  Points are read from
+// and results written to pinned host memory.
 struct LaneWords {
     const float* w;
     __device__ __forceinline__ float operator()(int k) const { return w[k]; }
@@ -485,6 +486,7 @@ __global__ __launch_bounds__(64) void k_point(const uint32_t* __restrict__ slots
     extern __shared__ uint4 nodebuf[]; // PF: one node per stack position (width * 6 uint4 each), lds_nodes of them
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
+    if (done && *(volatile const uint8_t*)(done + i) != 0) return; // answered by k_point_bfs
     const int nwords = width * (kSlotWords / 4); // uint4 per node
     const float qx = Q ? Q[3 * (size_t)i] : pa.q[i][0];
     const float qy = Q ? Q[3 * (size_t)i + 1] : pa.q[i][1];
@@ -592,6 +594,290 @@ __global__ __launch_bounds__(64) void k_point(const uint32_t* __restrict__ slots
             __threadfence_system();
             *(volatile uint8_t*)(done + i) = 1;
         }
+    }
+}
+
+// Breadth-first single-point walk (host calls of a few points; the per-point API).  The walk of
+// k_point is a chain of dependent node loads; here one wave per point takes the tree a level at
+// a time, every slot of up to 64 / width of the level's nodes in its own lane, so the point waits
+// for one load round trip per level step instead of one per node.  Each visited node gets a
+// record in LDS: per slot its kind (far, near internal, near FP32 leaf, near FP64 leaf), the far
+// term's t and inv3, the child's record or the FP32 leaf's triangle values.  FP64 near leaves
+// are evaluated on the way and summed in 2^-40 fixed point (integer addition: any order).
+//
+// The FP32 terms must then be added in the batched traversal's order: a node's own terms in slot
+// order (far terms, then the triangles of near leaves), followed by the subtrees of its near
+// children in reverse slot order (they were pushed in slot order and are popped last first).  So
+// two more level-parallel passes over the records compute, bottom up, every record's number of
+// terms T and, top down, where its terms start in that order (a slot's own terms after the
+// node's earlier own terms; a child's subtree after the node's own terms and the subtrees of the
+// children in later slots), writing each term's (a, b) location to that position of a list.
+// One chain of fmaf over the list gives the sum: far terms as fmaf(t, inv3, omega) and triangle
+// values as fmaf(1, v, omega) = omega + v (exact product, one rounding), the same operations on
+// the same values as the batched traversal, so a point gets bit for bit the value a batched call
+// gives it.  A walk that needs more than max_rec records, max_tv triangle values or kBfsLevels
+// levels writes no result and marks its done byte 2 (the host answers that point with k_point);
+// every other point marks it 1.
+constexpr uint32_t kRecLeaf64 = 0u, kRecFar = 1u, kRecInner = 2u, kRecLeaf32 = 3u;
+constexpr int kBfsLevels = 64;
+// slot entry: kind in bits 0-1; a near internal slot: child record << 2; an FP32 leaf:
+// (triangles - 1) << 2 | first value << 5
+
+// sums over aligned groups of w lanes (w a power of two, all lanes active): inclusive prefix,
+// inclusive suffix and total
+__device__ __forceinline__ int seg_prefix(int x, int ls, int w)
+{
+    for (int d = 1; d < w; d <<= 1) {
+        const int y = __shfl_up(x, d, w);
+        if (ls >= d) x += y;
+    }
+    return x;
+}
+__device__ __forceinline__ int seg_suffix(int x, int ls, int w)
+{
+    for (int d = 1; d < w; d <<= 1) {
+        const int y = __shfl_down(x, d, w);
+        if (ls + d < w) x += y;
+    }
+    return x;
+}
+__device__ __forceinline__ int seg_total(int x, int w)
+{
+    for (int d = 1; d < w; d <<= 1) x += __shfl_xor(x, d, w);
+    return x;
+}
+
+__global__ __launch_bounds__(64) void k_point_bfs(const uint32_t* __restrict__ slots, int width,
+                                                  const uint32_t* __restrict__ tris, int32_t root, int32_t root_ns,
+                                                  float tau, const float* __restrict__ Q, PointArgs pa, int max_rec,
+                                                  int max_tv, float* __restrict__ omega, uint8_t* __restrict__ inside,
+                                                  uint8_t* __restrict__ done)
+{
+    // LDS (words): per record its node link, T and start; per slot entry, a, b (the FP32 triangle
+    // values follow the slots' a and b as a = 1, b = value); the list of term locations
+    __shared__ int lvl[kBfsLevels + 1]; // level L holds records [lvl[L], lvl[L + 1])
+    extern __shared__ uint32_t recs[];
+    const int nsl = max_rec * width; // slot entries
+    uint32_t* rec_node = recs;
+    int* rec_T = reinterpret_cast<int*>(rec_node + max_rec);
+    int* rec_S = rec_T + max_rec;
+    uint32_t* rec_k = reinterpret_cast<uint32_t*>(rec_S + max_rec);
+    float* rec_a = reinterpret_cast<float*>(rec_k + nsl);
+    float* rec_b = rec_a + nsl + max_tv;
+    uint32_t* lin = reinterpret_cast<uint32_t*>(rec_b + nsl + max_tv);
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const float qx = Q ? Q[3 * (size_t)i] : pa.q[i][0];
+    const float qy = Q ? Q[3 * (size_t)i + 1] : pa.q[i][1];
+    const float qz = Q ? Q[3 * (size_t)i + 2] : pa.q[i][2];
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int per = 64 / width;       // nodes per step
+    const int ln = lane / width;      // this lane's node within the step
+    const int ls = lane - ln * width; // and slot
+    float om = 0.0f;
+    unsigned long long acc_lane = 0ull; // this lane's FP64 near leaves, 2^-40 fixed point
+    bool redo = false;
+    if (root >= 0) {
+        if (lane == 0) {
+            rec_node[0] = ((uint32_t)root << 4) | (uint32_t)root_ns;
+            rec_S[0] = 0;
+            lvl[0] = 0;
+        }
+        __syncthreads();
+        // pass 1: the walk, a level at a time
+        int b = 0, e = 1, nfree = 1, tvfree = 0, nlev = 0;
+        while (b < e) {
+            if (nlev == kBfsLevels) {
+                redo = true;
+                break;
+            }
+            for (int c = b; c < e; c += per) {
+                const int r = c + ln;
+                const bool in_r = ln < per && r < e;
+                const uint32_t ne = in_r ? rec_node[r] : 0u;
+                const bool act = in_r && ls < (int)(ne & 15u);
+                float w[FWN_W_LINK];
+                uint32_t link = FWN_LINK_EMPTY;
+                float dx = 0.0f, dy = 0.0f, dz = 0.0f, d2 = 0.0f, t = 0.0f, inv3 = 0.0f;
+                bool far = false;
+                if (act) {
+                    const uint4* src =
+                        reinterpret_cast<const uint4*>(slots + ((size_t)(ne >> 4) * (size_t)width + ls) * kSlotWords);
+                    uint32_t u[kSlotWords];
+#pragma unroll
+                    for (int k = 0; k < kSlotWords / 4; ++k) {
+                        const uint4 v = src[k];
+                        u[4 * k] = v.x;
+                        u[4 * k + 1] = v.y;
+                        u[4 * k + 2] = v.z;
+                        u[4 * k + 3] = v.w;
+                    }
+#pragma unroll
+                    for (int k = 0; k < FWN_W_LINK; ++k) w[k] = __uint_as_float(u[k]);
+                    link = u[FWN_W_LINK];
+                    dx = w[FWN_W_PX] - qx;
+                    dy = w[FWN_W_PY] - qy;
+                    dz = w[FWN_W_PZ] - qz;
+                    d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    far = d2 > w[FWN_W_THR];
+                    far_field_terms(LaneWords{w}, dx, dy, dz, d2, t, inv3);
+                }
+                const bool inner = act && !far && (int32_t)link >= 0;
+                const bool nleaf = act && !far && (int32_t)link < 0;
+                // a near leaf: its first triangle's record (vertices and the FP64 rule's
+                // thresholds) in one round trip
+                int32_t first = 0, tcnt = 0;
+                float tw[kTriWords];
+                bool use64 = false;
+                if (nleaf) {
+                    first = (int32_t)((link & ~FWN_LINK_LEAF) >> 3);
+                    tcnt = (int32_t)(link & 7u) + 1;
+                    const uint4* tp = reinterpret_cast<const uint4*>(tris + (size_t)first * kTriWords);
+#pragma unroll
+                    for (int k = 0; k < kTriWords / 4; ++k) {
+                        const uint4 v = tp[k];
+                        tw[4 * k] = __uint_as_float(v.x);
+                        tw[4 * k + 1] = __uint_as_float(v.y);
+                        tw[4 * k + 2] = __uint_as_float(v.z);
+                        tw[4 * k + 3] = __uint_as_float(v.w);
+                    }
+                    const float nd = fmaf(dx, w[FWN_W_NX], fmaf(dy, w[FWN_W_NY], dz * w[FWN_W_NZ]));
+                    use64 = tau > 0.0f && d2 < tw[FWN_T_NEAR2] && (tcnt > 1 || nd * nd < tw[FWN_T_PLANE]);
+                }
+                const bool use32 = nleaf && !use64;
+                // records of the near children and places for the FP32 values, in lane order
+                const uint64_t im = __ballot(inner);
+                const int cnt = __popcll(im);
+                const int c32 = use32 ? tcnt : 0; // 1..8: four bits
+                int pre = 0, tot = 0;
+#pragma unroll
+                for (int bit = 0; bit < 4; ++bit) {
+                    const uint64_t bm = __ballot((c32 >> bit) & 1);
+                    pre += __popcll(bm & lt_mask) << bit;
+                    tot += __popcll(bm) << bit;
+                }
+                if (nfree + cnt > max_rec || tvfree + tot > max_tv) {
+                    redo = true;
+                    break;
+                }
+                const int child = nfree + __popcll(im & lt_mask);
+                if (inner) rec_node[child] = link;
+                if (use32) {
+                    const int v0 = nsl + tvfree + pre;
+                    rec_a[v0] = 1.0f;
+                    rec_b[v0] = tri_f32(tw, qx, qy, qz);
+                    for (int32_t k = 1; k < tcnt; ++k) {
+                        rec_a[v0 + k] = 1.0f;
+                        rec_b[v0 + k] =
+                            tri_f32(reinterpret_cast<const float*>(tris + (size_t)(first + k) * kTriWords), qx, qy, qz);
+                    }
+                }
+                if (use64) {
+                    double s64 = tri_f64(tw, qx, qy, qz);
+                    for (int32_t k = 1; k < tcnt; ++k)
+                        s64 += tri_f64(reinterpret_cast<const float*>(tris + (size_t)(first + k) * kTriWords), qx, qy, qz);
+                    acc_lane += (unsigned long long)__double2ll_rn(s64 * kFixScale);
+                }
+                if (act) {
+                    const int k = r * width + ls;
+                    rec_k[k] = far ? kRecFar
+                                   : (inner ? (kRecInner | ((uint32_t)child << 2))
+                                            : (use32 ? (kRecLeaf32 | ((uint32_t)(tcnt - 1) << 2) |
+                                                        ((uint32_t)(tvfree + pre) << 5))
+                                                     : kRecLeaf64));
+                    rec_a[k] = t;
+                    rec_b[k] = inv3;
+                }
+                nfree += cnt;
+                tvfree += tot;
+            }
+            if (redo) break;
+            ++nlev;
+            if (lane == 0) lvl[nlev] = e; // the level just walked is [b, e); the next, [e, nfree)
+            __syncthreads(); // the level's records before the next level reads them
+            b = e;
+            e = nfree;
+        }
+        if (!redo) {
+            // pass 2, bottom up: T, the terms of a record's subtree
+            for (int L = nlev - 1; L >= 0; --L) {
+                const int lb = lvl[L], le = lvl[L + 1];
+                for (int c = lb; c < le; c += per) {
+                    const int r = c + ln;
+                    const bool in_r = ln < per && r < le;
+                    const int ns = in_r ? (int)(rec_node[r] & 15u) : 0;
+                    int n = 0;
+                    if (ls < ns) {
+                        const uint32_t kk = rec_k[r * width + ls];
+                        const uint32_t kind = kk & 3u;
+                        n = kind == kRecFar ? 1
+                                            : (kind == kRecLeaf32 ? (int)((kk >> 2) & 7u) + 1
+                                                                  : (kind == kRecInner ? rec_T[kk >> 2] : 0));
+                    }
+                    n = seg_total(n, width);
+                    if (in_r && ls == 0) rec_T[r] = n;
+                }
+                __syncthreads();
+            }
+            // pass 3, top down: where each record's terms start; each term's location in the list
+            for (int L = 0; L < nlev; ++L) {
+                const int lb = lvl[L], le = lvl[L + 1];
+                for (int c = lb; c < le; c += per) {
+                    const int r = c + ln;
+                    const bool in_r = ln < per && r < le;
+                    const int ns = in_r ? (int)(rec_node[r] & 15u) : 0;
+                    const int k = r * width + ls;
+                    const uint32_t kk = ls < ns ? rec_k[k] : 0u;
+                    const uint32_t kind = ls < ns ? (kk & 3u) : kRecLeaf64;
+                    const int own = kind == kRecFar ? 1 : (kind == kRecLeaf32 ? (int)((kk >> 2) & 7u) + 1 : 0);
+                    const int sub = kind == kRecInner ? rec_T[kk >> 2] : 0;
+                    const int own_incl = seg_prefix(own, ls, width);
+                    const int own_all = __shfl(own_incl, width - 1, width);
+                    const int sub_incl = seg_suffix(sub, ls, width);
+                    const int sr = in_r ? rec_S[r] : 0;
+                    if (kind == kRecFar) {
+                        lin[sr + own_incl - 1] = (uint32_t)k;
+                    } else if (kind == kRecLeaf32) {
+                        const int p0 = sr + own_incl - own;
+                        const uint32_t v0 = (uint32_t)nsl + (kk >> 5);
+                        for (int j = 0; j < own; ++j) lin[p0 + j] = v0 + (uint32_t)j;
+                    } else if (kind == kRecInner) {
+                        rec_S[kk >> 2] = sr + own_all + sub_incl - sub;
+                    }
+                }
+                __syncthreads();
+            }
+            // the sum, in list order: one fmaf per term
+            const int nt = min(rec_T[0], nsl + max_tv); // at most one term per slot entry and value
+            for (int base = 0; base < nt; base += 64) {
+                float ta = 0.0f, tb = 0.0f;
+                if (base + lane < nt) {
+                    const uint32_t x = lin[base + lane];
+                    ta = rec_a[x];
+                    tb = rec_b[x];
+                }
+                const int n = min(64, nt - base);
+                for (int j = 0; j < n; ++j) om = fmaf(readlane_f(ta, j), readlane_f(tb, j), om);
+            }
+        }
+    }
+    // the wave's FP64 sum
+    unsigned long long acc = 0ull;
+    for (uint64_t m = __ballot(acc_lane != 0ull); m != 0ull; m &= m - 1ull) {
+        const int l = (int)__builtin_ctzll(m);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)acc_lane, l);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(acc_lane >> 32), l);
+        acc += ((unsigned long long)hi << 32) | lo;
+    }
+    const long long v = (long long)acc;
+    if (v != 0) om = (float)((double)om + (double)v * (1.0 / kFixScale));
+    if (lane == 0) {
+        if (!redo) {
+            omega[i] = om;
+            if (inside) inside[i] = ((double)om / (4.0 * 3.14159265358979323846) > 0.5) ? 1 : 0;
+        }
+        __threadfence_system(); // the host waits on this byte: the results go first
+        *(volatile uint8_t*)(done + i) = redo ? 2 : 1;
     }
 }
 
@@ -859,27 +1145,30 @@ void run_queries_pipelined(Engine& e, const float* Q, int64_t nQ, float* omega, 
     }
 }
 
-// Host calls of at most kSmallBatch points on a single-mesh engine: one k_point launch (points in
-// its arguments up to kPointArgs, else read from the pinned stage) writes the results into the
-// pinned stage, one wait.
-// The LDS node buffers (one per stack position the tree can need) are used when they fit in
-// kPointLds bytes.  These calls record no device time.
+// Host calls of at most kSmallBatch points on a single-mesh engine: one k_point_bfs launch
+// (points in its arguments up to kPointArgs, else read from the pinned stage) writes the results
+// into the pinned stage and a done byte per point; the points it could not hold in LDS (done
+// byte 2) go to one k_point launch.  k_point's LDS node buffers (one per stack position the tree
+// can need) are used when they fit in kPointLds bytes.  These calls record no device time.
 constexpr size_t kPointLds = 32768;
+constexpr size_t kPointBfsLds = 65536 - 512; // k_point_bfs: records, FP32 triangle values, term list (+ level table)
+constexpr int kPointBfsValues = 1024;
 
 void run_points(Engine& e, const float* Q, int64_t nQ, float* omega, uint8_t* inside, int banks)
 {
     hipStream_t s = e.stream;
     const size_t pts_b = (sizeof(float) * 3 * (size_t)nQ + 15) & ~(size_t)15;
-    // stage: overflow flag (16 B), done bytes of argument calls (16 B), points, omega, inside
-    const size_t out_b = 32 + pts_b + (sizeof(float) + 1) * (size_t)nQ;
+    const size_t done_b = ((size_t)nQ + 15) & ~(size_t)15;
+    // stage: overflow flag (16 B), a done byte per point, points, omega, inside
+    const size_t out_b = 16 + done_b + pts_b + (sizeof(float) + 1) * (size_t)nQ;
     char* hs = host_stage(e, out_b);
     int32_t* hflag = reinterpret_cast<int32_t*>(hs);
     uint8_t* hdone = reinterpret_cast<uint8_t*>(hs + 16);
-    float* hq = reinterpret_cast<float*>(hs + 32);
-    float* hom = reinterpret_cast<float*>(hs + 32 + pts_b);
-    uint8_t* hin = reinterpret_cast<uint8_t*>(hs + 32 + pts_b + sizeof(float) * (size_t)nQ);
+    float* hq = reinterpret_cast<float*>(hs + 16 + done_b);
+    float* hom = reinterpret_cast<float*>(hs + 16 + done_b + pts_b);
+    uint8_t* hin = reinterpret_cast<uint8_t*>(hs + 16 + done_b + pts_b + sizeof(float) * (size_t)nQ);
     *hflag = 0;
-    std::memset(hdone, 0, 16);
+    std::memset(hdone, 0, (size_t)nQ);
     PointArgs pa;
     std::memset(&pa, 0, sizeof(pa));
     const float* dq = hq; // points read from the pinned stage, or from the arguments (nullptr)
@@ -891,27 +1180,13 @@ void run_points(Engine& e, const float* Q, int64_t nQ, float* omega, uint8_t* in
     }
     const int32_t root = e.h_roots[0], root_ns = e.h_roots[1];
     const float tau = e.h_tau[0];
-    const size_t need = (size_t)(1 + (e.width - 1) * std::max(e.max_depth, 1)); // stack entries
-    const size_t lds = need * (size_t)e.width * kSlotWords * sizeof(uint32_t);
-    const bool pf = banks == 1 && lds <= kPointLds;
     const dim3 grid((unsigned)nQ), block(64);
-    // argument calls (the per-point API) wait by polling the done bytes the kernel writes after
-    // its results: no stream synchronisation in the round trip.  Every 256 polls the stream is
-    // asked whether it has finished, so a failed launch ends the wait.
-    uint8_t* done = dq ? nullptr : hdone;
-    if (pf)
-        hipLaunchKernelGGL((k_point<1, true>), grid, block, lds, s, e.slots.as<uint32_t>(), e.width,
-                           e.tris.as<uint32_t>(), root, root_ns, tau, dq, pa, hom, hin, hflag, (int)need, done);
-    else if (banks == 1)
-        hipLaunchKernelGGL((k_point<1, false>), grid, block, 0, s, e.slots.as<uint32_t>(), e.width,
-                           e.tris.as<uint32_t>(), root, root_ns, tau, dq, pa, hom, hin, hflag, 0, done);
-    else
-        hipLaunchKernelGGL((k_point<3, false>), grid, block, 0, s, e.slots.as<uint32_t>(), e.width,
-                           e.tris.as<uint32_t>(), root, root_ns, tau, dq, pa, hom, hin, hflag, 0, done);
-    FWN_LAUNCH_CHECK("k_point");
-    bool polled = false;
-    if (done) {
+    // the results are waited for by polling the done bytes the kernels write after them: no
+    // stream synchronisation in the round trip.  Every 256 polls the stream is asked whether it
+    // has finished, so a failed launch ends the wait.
+    auto wait_done = [&]() {
         const volatile uint8_t* vd = hdone;
+        bool polled = false;
         for (uint32_t spin = 1;; ++spin) {
             int64_t k = 0;
             while (k < nQ && vd[k] != 0) ++k;
@@ -922,8 +1197,39 @@ void run_points(Engine& e, const float* Q, int64_t nQ, float* omega, uint8_t* in
             if ((spin & 255u) == 0u && hipStreamQuery(s) != hipErrorNotReady) break;
         }
         std::atomic_thread_fence(std::memory_order_acquire);
+        if (!polled) FWN_HIP(hipStreamSynchronize(s));
+    };
+    // breadth-first walk, records in LDS
+    const int max_tv = kPointBfsValues;
+    const int max_rec = (int)((kPointBfsLds / sizeof(uint32_t) - 3 * (size_t)max_tv) / (size_t)(3 + 4 * e.width));
+    hipLaunchKernelGGL(k_point_bfs, grid, block, kPointBfsLds, s, e.slots.as<uint32_t>(), e.width,
+                       e.tris.as<uint32_t>(), root, root_ns, tau, dq, pa, max_rec, max_tv, hom, hin, hdone);
+    FWN_LAUNCH_CHECK("k_point_bfs");
+    wait_done();
+    // points whose walk needed more records than LDS holds: the depth-first kernel, which skips
+    // the points already answered (done byte 1)
+    int64_t n_redo = 0;
+    for (int64_t k = 0; k < nQ; ++k)
+        if (hdone[k] == 2) {
+            hdone[k] = 0;
+            ++n_redo;
+        }
+    if (n_redo > 0) {
+        const size_t need = (size_t)(1 + (e.width - 1) * std::max(e.max_depth, 1)); // stack entries
+        const size_t lds = need * (size_t)e.width * kSlotWords * sizeof(uint32_t);
+        const bool pf = banks == 1 && lds <= kPointLds;
+        if (pf)
+            hipLaunchKernelGGL((k_point<1, true>), grid, block, lds, s, e.slots.as<uint32_t>(), e.width,
+                               e.tris.as<uint32_t>(), root, root_ns, tau, dq, pa, hom, hin, hflag, (int)need, hdone);
+        else if (banks == 1)
+            hipLaunchKernelGGL((k_point<1, false>), grid, block, 0, s, e.slots.as<uint32_t>(), e.width,
+                               e.tris.as<uint32_t>(), root, root_ns, tau, dq, pa, hom, hin, hflag, 0, hdone);
+        else
+            hipLaunchKernelGGL((k_point<3, false>), grid, block, 0, s, e.slots.as<uint32_t>(), e.width,
+                               e.tris.as<uint32_t>(), root, root_ns, tau, dq, pa, hom, hin, hflag, 0, hdone);
+        FWN_LAUNCH_CHECK("k_point");
+        wait_done();
     }
-    if (!polled) FWN_HIP(hipStreamSynchronize(s));
     if (*hflag) fail(FWN_LIMIT_EXCEEDED, "traversal stack overflow");
     std::memcpy(omega, hom, sizeof(float) * (size_t)nQ);
     if (inside) std::memcpy(inside, hin, (size_t)nQ);

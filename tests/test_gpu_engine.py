@@ -430,14 +430,25 @@ def _telescope_soup(n=100):
     return tri.reshape(-1, 3), np.arange(3 * n, dtype=np.int32).reshape(-1, 3)
 
 
-@pytest.mark.parametrize("case", ["ico5_fp64", "open_leaf4", "width2", "width8_lbvh", "deep"])
-def test_point_kernel_equals_batched(ico5, open_mesh, case):
-    """Host calls of at most 2,048 points on a single-mesh engine run the single-point kernel (one
-    wave per point, the node's slots in parallel lanes, children fetched into LDS as they are
-    pushed, terms added in the batched traversal's order): every point equals the batched call
-    bit for bit, Omega and flag, in calls of 1 to 16 points and of 300 and 2,048 points, with FP64
-    near leaves, leaf buckets, tree widths 2 / 8, the Morton builder and a tree deep enough for the
-    three-bank stack (which reads its nodes without the LDS buffers)."""
+def _pile(n=6000):
+    """Large random triangles overlapping in the cube [-1, 1]^3: a point inside the pile is near
+    almost every node, so its walk visits most of the tree."""
+    rng = np.random.default_rng(9)
+    V = rng.uniform(-1, 1, (3 * n, 3)).astype(np.float32)
+    return V, np.arange(3 * n, dtype=np.int32).reshape(-1, 3)
+
+
+@pytest.mark.parametrize("case", ["ico5_fp64", "open_leaf4", "width2", "width8_lbvh", "deep", "records_overflow"])
+def test_point_kernel_equals_batched(oracle, ico5, open_mesh, case):
+    """Host calls of at most 2,048 points on a single-mesh engine run the breadth-first
+    single-point kernel (one wave per point, a level of the tree at a time with the slots of
+    several nodes in parallel lanes, per-node records in LDS, then the terms added depth first in
+    the batched traversal's order); a point whose walk needs more records or FP32 triangle values
+    than LDS holds is answered by the depth-first kernel (children fetched into LDS as they are
+    pushed).  Every point equals the batched call bit for bit, Omega and flag, in calls of 1 to 16
+    points and of 300 and 2,048 points, with FP64 near leaves, leaf buckets, tree widths 2 / 8,
+    the Morton builder, a tree deep enough for the three-bank stack and a pile of overlapping
+    triangles whose inner points overflow the records (mixed with outer points in one call)."""
     opts = {}
     if case == "ico5_fp64":
         V, F = ico5
@@ -450,6 +461,8 @@ def test_point_kernel_equals_batched(ico5, open_mesh, case):
     elif case == "width8_lbvh":
         V, F = open_mesh
         opts = dict(tree_width=8, tree_builder=L.FWN_TREE_LBVH, leaf_max=2)
+    elif case == "records_overflow":
+        V, F = _pile()
     else:
         V, F = _telescope_soup()
     e = FastWindingNumber((V, F), **opts)
@@ -459,6 +472,18 @@ def test_point_kernel_equals_batched(ico5, open_mesh, case):
     lo, hi = V.min(0), V.max(0)
     if case == "deep":
         q = (rng.standard_normal((300, 3)) * (0.5 ** rng.integers(0, 40, 300))[:, None] * 3).astype(np.float32)
+    elif case == "records_overflow":
+        inner = rng.uniform(-0.3, 0.3, (150, 3))
+        outer = rng.standard_normal((250, 3))
+        outer *= (4.0 + 20.0 * rng.random(250))[:, None] / np.linalg.norm(outer, axis=1, keepdims=True)
+        q = np.concatenate([inner, outer])[rng.permutation(400)].astype(np.float32)
+        # the restated tree's walk (same tree and rule) on the CPU: the inner points test more
+        # slots than the records hold (16,384 - 1,024 words at 13 per 4-wide node) or add more
+        # than 1,024 FP32 triangle values; the outer points stop at the first levels
+        _, _, _, tau = e.export_tree()
+        _, rst = oracle.Restated(V, F).solid_angle(q, 2.0, tau=float(tau[0]), stats=True)
+        over = (rst[:, 0] > 4 * ((16384 - 1024) // 13)) | (rst[:, 2] > 1024)
+        assert 100 <= over.sum() <= 200, over.sum()
     else:
         q = np.concatenate([W.uniform_queries(lo, hi, 200, 3, 1.3),
                             W.near_surface_queries(W.Mesh(V.astype(np.float64), F.astype(np.int64)), 200, 4)])
