@@ -611,8 +611,8 @@ __global__ __launch_bounds__(64) void k_point(const uint32_t* __restrict__ slots
 // two more level-parallel passes over the records compute, bottom up, every record's number of
 // terms T and, top down, where its terms start in that order (a slot's own terms after the
 // node's earlier own terms; a child's subtree after the node's own terms and the subtrees of the
-// children in later slots), writing each term's (a, b) location to that position of a list.
-// One chain of fmaf over the list gives the sum: far terms as fmaf(t, inv3, omega) and triangle
+// children in later slots), writing each term's pair (a, b) to that position of a list.  One
+// chain of fmaf over the list gives the sum: far terms as fmaf(t, inv3, omega) and triangle
 // values as fmaf(1, v, omega) = omega + v (exact product, one rounding), the same operations on
 // the same values as the batched traversal, so a point gets bit for bit the value a batched call
 // gives it.  A walk that needs more than max_rec records, max_tv triangle values or kBfsLevels
@@ -623,27 +623,39 @@ constexpr int kBfsLevels = 64;
 // slot entry: kind in bits 0-1; a near internal slot: child record << 2; an FP32 leaf:
 // (triangles - 1) << 2 | first value << 5
 
-// sums over aligned groups of w lanes (w a power of two, all lanes active): inclusive prefix,
-// inclusive suffix and total
+// sums over aligned groups of w lanes (w = 2, 4 or 8, all lanes active; ls = lane % w): inclusive
+// prefix and inclusive suffix by DPP row shifts (a group lies inside one 16-lane row, and the
+// lanes a shift brings in from outside the group are masked off)
+template <int D>
+__device__ __forceinline__ int dpp_shr(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x110 + D, 0xF, 0xF, true); }
+template <int D>
+__device__ __forceinline__ int dpp_shl(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x100 + D, 0xF, 0xF, true); }
 __device__ __forceinline__ int seg_prefix(int x, int ls, int w)
 {
-    for (int d = 1; d < w; d <<= 1) {
-        const int y = __shfl_up(x, d, w);
-        if (ls >= d) x += y;
+    int y = dpp_shr<1>(x);
+    x += ls >= 1 ? y : 0;
+    if (w > 2) {
+        y = dpp_shr<2>(x);
+        x += ls >= 2 ? y : 0;
+    }
+    if (w > 4) {
+        y = dpp_shr<4>(x);
+        x += ls >= 4 ? y : 0;
     }
     return x;
 }
 __device__ __forceinline__ int seg_suffix(int x, int ls, int w)
 {
-    for (int d = 1; d < w; d <<= 1) {
-        const int y = __shfl_down(x, d, w);
-        if (ls + d < w) x += y;
+    int y = dpp_shl<1>(x);
+    x += ls + 1 < w ? y : 0;
+    if (w > 2) {
+        y = dpp_shl<2>(x);
+        x += ls + 2 < w ? y : 0;
     }
-    return x;
-}
-__device__ __forceinline__ int seg_total(int x, int w)
-{
-    for (int d = 1; d < w; d <<= 1) x += __shfl_xor(x, d, w);
+    if (w > 4) {
+        y = dpp_shl<4>(x);
+        x += ls + 4 < w ? y : 0;
+    }
     return x;
 }
 
@@ -653,18 +665,19 @@ __global__ __launch_bounds__(64) void k_point_bfs(const uint32_t* __restrict__ s
                                                   int max_tv, float* __restrict__ omega, uint8_t* __restrict__ inside,
                                                   uint8_t* __restrict__ done)
 {
-    // LDS (words): per record its node link, T and start; per slot entry, a, b (the FP32 triangle
-    // values follow the slots' a and b as a = 1, b = value); the list of term locations
+    // LDS: the list of term pairs (at most one per slot entry and FP32 value); per record its
+    // node link, T and start; per slot entry its kind word (kind 0 for a node's unused slots),
+    // t and inv3; the FP32 triangle values
     __shared__ int lvl[kBfsLevels + 1]; // level L holds records [lvl[L], lvl[L + 1])
-    extern __shared__ uint32_t recs[];
+    extern __shared__ float2 terms[];
     const int nsl = max_rec * width; // slot entries
-    uint32_t* rec_node = recs;
+    uint32_t* rec_node = reinterpret_cast<uint32_t*>(terms + nsl + max_tv);
     int* rec_T = reinterpret_cast<int*>(rec_node + max_rec);
     int* rec_S = rec_T + max_rec;
     uint32_t* rec_k = reinterpret_cast<uint32_t*>(rec_S + max_rec);
     float* rec_a = reinterpret_cast<float*>(rec_k + nsl);
-    float* rec_b = rec_a + nsl + max_tv;
-    uint32_t* lin = reinterpret_cast<uint32_t*>(rec_b + nsl + max_tv);
+    float* rec_b = rec_a + nsl;
+    float* trival = rec_b + nsl;
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     const float qx = Q ? Q[3 * (size_t)i] : pa.q[i][0];
@@ -763,14 +776,10 @@ __global__ __launch_bounds__(64) void k_point_bfs(const uint32_t* __restrict__ s
                 const int child = nfree + __popcll(im & lt_mask);
                 if (inner) rec_node[child] = link;
                 if (use32) {
-                    const int v0 = nsl + tvfree + pre;
-                    rec_a[v0] = 1.0f;
-                    rec_b[v0] = tri_f32(tw, qx, qy, qz);
-                    for (int32_t k = 1; k < tcnt; ++k) {
-                        rec_a[v0 + k] = 1.0f;
-                        rec_b[v0 + k] =
-                            tri_f32(reinterpret_cast<const float*>(tris + (size_t)(first + k) * kTriWords), qx, qy, qz);
-                    }
+                    float* tv = trival + tvfree + pre;
+                    tv[0] = tri_f32(tw, qx, qy, qz);
+                    for (int32_t k = 1; k < tcnt; ++k)
+                        tv[k] = tri_f32(reinterpret_cast<const float*>(tris + (size_t)(first + k) * kTriWords), qx, qy, qz);
                 }
                 if (use64) {
                     double s64 = tri_f64(tw, qx, qy, qz);
@@ -778,9 +787,9 @@ __global__ __launch_bounds__(64) void k_point_bfs(const uint32_t* __restrict__ s
                         s64 += tri_f64(reinterpret_cast<const float*>(tris + (size_t)(first + k) * kTriWords), qx, qy, qz);
                     acc_lane += (unsigned long long)__double2ll_rn(s64 * kFixScale);
                 }
-                if (act) {
+                if (in_r) {
                     const int k = r * width + ls;
-                    rec_k[k] = far ? kRecFar
+                    rec_k[k] = !act ? kRecLeaf64 : far ? kRecFar
                                    : (inner ? (kRecInner | ((uint32_t)child << 2))
                                             : (use32 ? (kRecLeaf32 | ((uint32_t)(tcnt - 1) << 2) |
                                                         ((uint32_t)(tvfree + pre) << 5))
@@ -805,16 +814,15 @@ __global__ __launch_bounds__(64) void k_point_bfs(const uint32_t* __restrict__ s
                 for (int c = lb; c < le; c += per) {
                     const int r = c + ln;
                     const bool in_r = ln < per && r < le;
-                    const int ns = in_r ? (int)(rec_node[r] & 15u) : 0;
                     int n = 0;
-                    if (ls < ns) {
+                    if (in_r) {
                         const uint32_t kk = rec_k[r * width + ls];
                         const uint32_t kind = kk & 3u;
                         n = kind == kRecFar ? 1
                                             : (kind == kRecLeaf32 ? (int)((kk >> 2) & 7u) + 1
                                                                   : (kind == kRecInner ? rec_T[kk >> 2] : 0));
                     }
-                    n = seg_total(n, width);
+                    n = seg_prefix(n, ls, width) + seg_suffix(n, ls, width) - n; // the group's total
                     if (in_r && ls == 0) rec_T[r] = n;
                 }
                 __syncthreads();
@@ -825,39 +833,48 @@ __global__ __launch_bounds__(64) void k_point_bfs(const uint32_t* __restrict__ s
                 for (int c = lb; c < le; c += per) {
                     const int r = c + ln;
                     const bool in_r = ln < per && r < le;
-                    const int ns = in_r ? (int)(rec_node[r] & 15u) : 0;
                     const int k = r * width + ls;
-                    const uint32_t kk = ls < ns ? rec_k[k] : 0u;
-                    const uint32_t kind = ls < ns ? (kk & 3u) : kRecLeaf64;
+                    uint32_t kk = 0u;
+                    int sr = 0;
+                    float ta = 0.0f, tb = 0.0f;
+                    if (in_r) {
+                        kk = rec_k[k];
+                        sr = rec_S[r];
+                        ta = rec_a[k];
+                        tb = rec_b[k];
+                    }
+                    const uint32_t kind = kk & 3u;
                     const int own = kind == kRecFar ? 1 : (kind == kRecLeaf32 ? (int)((kk >> 2) & 7u) + 1 : 0);
                     const int sub = kind == kRecInner ? rec_T[kk >> 2] : 0;
                     const int own_incl = seg_prefix(own, ls, width);
-                    const int own_all = __shfl(own_incl, width - 1, width);
+                    const int own_all = own_incl + seg_suffix(own, ls, width) - own; // the node's own terms
                     const int sub_incl = seg_suffix(sub, ls, width);
-                    const int sr = in_r ? rec_S[r] : 0;
                     if (kind == kRecFar) {
-                        lin[sr + own_incl - 1] = (uint32_t)k;
+                        terms[sr + own_incl - 1] = make_float2(ta, tb);
                     } else if (kind == kRecLeaf32) {
                         const int p0 = sr + own_incl - own;
-                        const uint32_t v0 = (uint32_t)nsl + (kk >> 5);
-                        for (int j = 0; j < own; ++j) lin[p0 + j] = v0 + (uint32_t)j;
+                        const float* tv = trival + (kk >> 5);
+                        for (int j = 0; j < own; ++j) terms[p0 + j] = make_float2(1.0f, tv[j]);
                     } else if (kind == kRecInner) {
                         rec_S[kk >> 2] = sr + own_all + sub_incl - sub;
                     }
                 }
                 __syncthreads();
             }
-            // the sum, in list order: one fmaf per term
+            // the sum, in list order: one fmaf per term (every lane runs the chain on broadcast
+            // reads, 16 reads in flight ahead of their fmaf)
             const int nt = min(rec_T[0], nsl + max_tv); // at most one term per slot entry and value
-            for (int base = 0; base < nt; base += 64) {
-                float ta = 0.0f, tb = 0.0f;
-                if (base + lane < nt) {
-                    const uint32_t x = lin[base + lane];
-                    ta = rec_a[x];
-                    tb = rec_b[x];
-                }
-                const int n = min(64, nt - base);
-                for (int j = 0; j < n; ++j) om = fmaf(readlane_f(ta, j), readlane_f(tb, j), om);
+            int j = 0;
+            for (; j + 16 <= nt; j += 16) {
+                float2 v[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) v[u] = terms[j + u];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) om = fmaf(v[u].x, v[u].y, om);
+            }
+            for (; j < nt; ++j) {
+                const float2 v = terms[j];
+                om = fmaf(v.x, v.y, om);
             }
         }
     }
@@ -1201,7 +1218,7 @@ void run_points(Engine& e, const float* Q, int64_t nQ, float* omega, uint8_t* in
     };
     // breadth-first walk, records in LDS
     const int max_tv = kPointBfsValues;
-    const int max_rec = (int)((kPointBfsLds / sizeof(uint32_t) - 3 * (size_t)max_tv) / (size_t)(3 + 4 * e.width));
+    const int max_rec = (int)((kPointBfsLds / sizeof(uint32_t) - 3 * (size_t)max_tv) / (size_t)(3 + 5 * e.width));
     hipLaunchKernelGGL(k_point_bfs, grid, block, kPointBfsLds, s, e.slots.as<uint32_t>(), e.width,
                        e.tris.as<uint32_t>(), root, root_ns, tau, dq, pa, max_rec, max_tv, hom, hin, hdone);
     FWN_LAUNCH_CHECK("k_point_bfs");
