@@ -534,10 +534,12 @@ def bench_surrogate(mesh, steps: int = 3, train_steps: int = 0):
         "fp64_sample": int(idx.numel()),
         "train_s": si["train_ms"] / 1e3,
         "train_loss": si["last_loss"],
-        "encode_ms": si["encode_ms"],
-        "mlp_ms": si["mlp_ms"],
+        "sort_ms": si["encode_ms"],
+        "encode_mlp_ms": si["mlp_ms"],
         "fallback_ms": si["fallback_ms"],
-        "mfma": {"kernel": "k_sg_mlp", "achieved": mlp_flops / (si["mlp_ms"] * 1e-3) / 1e12, "peak": 2500.0,
+        # k_sg_infer fuses the hash-grid encoding with the MLP: the MLP's FLOPs over the fused
+        # kernel's time (encoding included)
+        "mfma": {"kernel": "k_sg_infer", "achieved": mlp_flops / (si["mlp_ms"] * 1e-3) / 1e12, "peak": 2500.0,
                  "unit": "TFLOP/s", "frac": mlp_flops / (si["mlp_ms"] * 1e-3) / 1e12 / 2500.0},
     }
 

@@ -331,8 +331,8 @@ typedef struct fwn_surrogate_info {
     float delta;           /* absolute distance bound */
     float last_loss;       /* mean squared error of the last training batch */
     double train_ms;       /* sampling + labelling + training, device time */
-    double encode_ms;      /* last query / predict call: hash-grid encoding kernel */
-    double mlp_ms;         /* last query / predict call: fused MLP kernel */
+    double encode_ms;      /* last query / predict call: Morton sort of the points (batches of 65,536 or more) */
+    double mlp_ms;         /* last query / predict call: fused encoding + MLP kernel */
     double fallback_ms;    /* last query call: compaction + exact engine on the fallback points */
     double query_ms;       /* last query call: whole call on the device */
 } fwn_surrogate_info;

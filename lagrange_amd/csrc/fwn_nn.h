@@ -48,6 +48,7 @@ __device__ __forceinline__ int acc_row(int r, int lane) { return 8 * (r >> 2) + 
 struct GridLevel {
     int32_t res;   // cells per axis; vertices 0..res
     int32_t dense; // 1: index = x + y (res+1) + z (res+1)^2
+    float rf, rf1; // res and res - 1 as floats (kernel arguments: no conversion per point)
 };
 
 __device__ __forceinline__ uint32_t grid_index(const GridLevel& g, uint32_t x, uint32_t y, uint32_t z, uint32_t mask)
@@ -67,10 +68,10 @@ struct Cell {
 __device__ __forceinline__ Cell grid_cell(const GridLevel& g, float px, float py, float pz)
 {
     Cell c;
-    const float r = (float)g.res;
+    const float r = g.rf;
     float sx = fminf(fmaxf(px, 0.0f), 1.0f) * r, sy = fminf(fmaxf(py, 0.0f), 1.0f) * r,
           sz = fminf(fmaxf(pz, 0.0f), 1.0f) * r;
-    float ix = fminf(floorf(sx), r - 1.0f), iy = fminf(floorf(sy), r - 1.0f), iz = fminf(floorf(sz), r - 1.0f);
+    float ix = fminf(floorf(sx), g.rf1), iy = fminf(floorf(sy), g.rf1), iz = fminf(floorf(sz), g.rf1);
     c.x = (uint32_t)ix;
     c.y = (uint32_t)iy;
     c.z = (uint32_t)iz;
