@@ -25,6 +25,7 @@ constexpr int RS_THREADS = 256;
 constexpr int RS_ROUNDS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;
 constexpr int RS_WAVES = RS_THREADS / kWave;
+typedef __attribute__((address_space(3))) volatile int32_t lds_vint;
 
 // lanes of the wave holding the same 8-bit digit d as this lane (among the lanes in `live`, a
 // prefix of the wave).  Morton keys of nearby points often share a digit across the whole wave:
@@ -181,8 +182,18 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const KeyT* __restric
     for (int r = 0; r < RS_ROUNDS; ++r) {
         const int li = wave * kWaveKeys + r * kWave + lane;
         kr[r] = (li < tile_n) ? kin[base + li] : (KeyT)0;
-        // vin == nullptr: the values are the input positions (a first pass over unsorted keys)
-        vr[r] = (li < tile_n) ? (vin ? vin[base + li] : (uint32_t)(base + li)) : 0u;
+    }
+    // vin == nullptr: the values are the input positions (a first pass over unsorted keys); the
+    // test is outside the loops so that no round waits for the loads of the others
+    if (vin) {
+#pragma unroll
+        for (int r = 0; r < RS_ROUNDS; ++r) {
+            const int li = wave * kWaveKeys + r * kWave + lane;
+            vr[r] = (li < tile_n) ? vin[base + li] : 0u;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < RS_ROUNDS; ++r) vr[r] = (uint32_t)(base + wave * kWaveKeys + r * kWave + lane);
     }
     // this tile's count of digit tid: the next block's exclusive offset (or the digit total) minus ours
     const int32_t mine = hist[(int64_t)tid * nblocks + blockIdx.x];
@@ -206,7 +217,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const KeyT* __restric
     // each wave: the place of every key in the wave's run of its digit
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     // volatile: each round reloads the counters other lanes of the wave wrote in earlier rounds
-    volatile int32_t* run_of = wrun[wave];
+    // (an LDS pointer, so that the accesses are LDS instructions rather than flat ones)
+    lds_vint* run_of = (lds_vint*)wrun[wave];
     int32_t lp[RS_ROUNDS];
 #pragma unroll
     for (int r = 0; r < RS_ROUNDS; ++r) {
