@@ -4,6 +4,7 @@
 // (modules/winding/src/FastWindingNumber.cpp:36-37, 91-96), which throw lagrange::Error.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -11,6 +12,7 @@
 #include <new>
 #include <sstream>
 #include <string>
+#include <thread>
 
 #include "fwn_internal.h"
 
@@ -64,21 +66,46 @@ Engine* make_engine(const fwn_options* opt)
 // on one engine are coalesced: each request joins the pending list; a caller that finds no
 // batch running takes the whole list, answers it with one batched query (results do not depend
 // on which points share a batch) and wakes the others.  Throughput therefore grows with the
-// number of calling threads instead of being one GPU round trip per point.
+// number of calling threads instead of being one GPU round trip per point.  Callers waiting for
+// a running batch poll their result for up to kPointSpinUs before they sleep, so they are back
+// with their next request when the batch ends; a new batch first waits (at most
+// kPointGatherUs) until as many requests are pending as the previous batch held.  A lone caller
+// (previous batch of one) never waits, and a wait that times out lowers the expectation to what
+// arrived.
+constexpr int kPointGatherUs = 10;
+constexpr int kPointSpinUs = 200;
+
 void point_query(Engine& e, const float q[3], float* omega, uint8_t* inside)
 {
     PointRequest r;
     std::memcpy(r.q, q, sizeof(r.q));
     std::unique_lock<std::mutex> lk(e.pt_mu);
     e.pt_pending.push_back(&r);
-    while (!r.done) {
+    while (!r.done.load(std::memory_order_acquire)) {
         if (e.pt_busy) {
-            e.pt_cv.wait(lk);
+            // a batch is running: poll for this request's result without the lock (a woken
+            // sleeper would come back too late for the next batch), then sleep until it is done
+            // or the engine is free
+            lk.unlock();
+            const auto t_end = std::chrono::steady_clock::now() + std::chrono::microseconds(kPointSpinUs);
+            while (!r.done.load(std::memory_order_acquire) && std::chrono::steady_clock::now() < t_end)
+                std::this_thread::yield();
+            lk.lock();
+            e.pt_cv.wait(lk, [&] { return r.done.load(std::memory_order_acquire) || !e.pt_busy; });
             continue;
         }
         e.pt_busy = true;
+        if (e.pt_pending.size() < e.pt_conc) {
+            const auto t_end = std::chrono::steady_clock::now() + std::chrono::microseconds(kPointGatherUs);
+            while (e.pt_pending.size() < e.pt_conc && std::chrono::steady_clock::now() < t_end) {
+                lk.unlock();
+                std::this_thread::yield();
+                lk.lock();
+            }
+        }
         std::vector<PointRequest*> batch;
         batch.swap(e.pt_pending);
+        e.pt_conc = batch.size();
         lk.unlock();
         const size_t n = batch.size();
         int st = FWN_OK;
@@ -106,7 +133,7 @@ void point_query(Engine& e, const float q[3], float* omega, uint8_t* inside)
             batch[k]->inside = in[k];
             batch[k]->status = st;
             batch[k]->err = msg;
-            batch[k]->done = true;
+            batch[k]->done.store(true, std::memory_order_release);
         }
         e.pt_busy = false;
         e.pt_cv.notify_all();

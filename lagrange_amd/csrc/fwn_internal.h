@@ -5,6 +5,7 @@
 
 #include <condition_variable>
 #include <cstdint>
+#include <atomic>
 #include <mutex>
 #include <new>
 #include <string>
@@ -65,7 +66,7 @@ struct PointRequest {
     uint8_t inside = 0;
     int status = 0;
     std::string err;
-    bool done = false;
+    std::atomic<bool> done{false}; // set (release) after the fields above, under the engine's pt_mu
 };
 
 struct Engine {
@@ -78,6 +79,7 @@ struct Engine {
     std::condition_variable pt_cv;
     std::vector<PointRequest*> pt_pending;
     bool pt_busy = false;
+    size_t pt_conc = 1; // requests the last coalesced batch held (how many callers to expect)
     fwn_options opt{};
 
     int32_t n_meshes = 0;
