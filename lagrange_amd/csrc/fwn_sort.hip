@@ -133,21 +133,6 @@ __global__ __launch_bounds__(256) void k_rs_rows(int32_t* __restrict__ hist, int
     if (tid == 0) rowsum[blockIdx.x] = carry;
 }
 
-__global__ __launch_bounds__(256) void k_rs_digits(int32_t* __restrict__ rowsum)
-{
-    __shared__ int32_t part[256];
-    int32_t v = rowsum[threadIdx.x];
-    part[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        int32_t t = (threadIdx.x >= (unsigned)off) ? part[threadIdx.x - off] : 0;
-        __syncthreads();
-        part[threadIdx.x] += t;
-        __syncthreads();
-    }
-    rowsum[threadIdx.x] = part[threadIdx.x] - v;
-}
-
 // Wave w of the block owns keys [w * 1024, (w + 1) * 1024) of the tile.  Each wave ranks its
 // keys on its own (no block barrier per round): per round of 64 keys the 8 digit ballots give a
 // key's rank among the wave's equal digits, and wave-private run counters in LDS give its place
@@ -161,7 +146,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const KeyT* __restric
                                                            const uint32_t* __restrict__ vin,
                                                            KeyT* __restrict__ kout, uint32_t* __restrict__ vout,
                                                            int64_t n, int shift, const int32_t* __restrict__ hist,
-                                                           const int32_t* __restrict__ digit_base, int nblocks)
+                                                           const int32_t* __restrict__ digit_total, int nblocks)
 {
     constexpr int kWaveKeys = RS_TILE / RS_WAVES; // 1024
     __shared__ int32_t gbase[256];          // global position of this tile's first key of each digit
@@ -196,24 +181,27 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const KeyT* __restric
         for (int r = 0; r < RS_ROUNDS; ++r) vr[r] = (uint32_t)(base + wave * kWaveKeys + r * kWave + lane);
     }
     // this tile's count of digit tid: the next block's exclusive offset (or the digit total) minus ours
+    const int32_t tot = digit_total[tid];
     const int32_t mine = hist[(int64_t)tid * nblocks + blockIdx.x];
-    int32_t next;
-    if (blockIdx.x + 1 < nblocks) next = hist[(int64_t)tid * nblocks + blockIdx.x + 1];
-    else next = ((tid < 255) ? digit_base[tid + 1] : (int32_t)n) - digit_base[tid];
+    const int32_t next = (blockIdx.x + 1 < nblocks) ? hist[(int64_t)tid * nblocks + blockIdx.x + 1] : tot;
     const int32_t cnt = next - mine;
-    gbase[tid] = mine + digit_base[tid];
 #pragma unroll
     for (int w = 0; w < RS_WAVES; ++w) wrun[w][tid] = 0;
-    // exclusive scan of the tile's digit counts (Hillis-Steele in LDS)
+    // exclusive scans (Hillis-Steele in LDS) of the tile's digit counts and of the digit totals
+    // (every tile scans the 256 totals itself: no separate launch)
     toff[tid] = cnt;
+    gbase[tid] = tot;
     __syncthreads();
     for (int off = 1; off < 256; off <<= 1) {
         const int32_t t = (tid >= off) ? toff[tid - off] : 0;
+        const int32_t u = (tid >= off) ? gbase[tid - off] : 0;
         __syncthreads();
         toff[tid] += t;
+        gbase[tid] += u;
         __syncthreads();
     }
     toff[tid] -= cnt;
+    gbase[tid] += mine - tot; // keys of smaller digits, then this digit's keys in earlier tiles
     // each wave: the place of every key in the wave's run of its digit
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     // volatile: each round reloads the counters other lanes of the wave wrote in earlier rounds
@@ -355,8 +343,6 @@ void radix_sort_core(KeyT*& keys, uint32_t*& vals, KeyT*& keys_alt, uint32_t*& v
         FWN_LAUNCH_CHECK("k_rs_hist");
         hipLaunchKernelGGL(k_rs_rows, dim3(256), dim3(256), 0, s, hist, nblocks, rowsum);
         FWN_LAUNCH_CHECK("k_rs_rows");
-        hipLaunchKernelGGL(k_rs_digits, dim3(1), dim3(256), 0, s, rowsum);
-        FWN_LAUNCH_CHECK("k_rs_digits");
         const uint32_t* vin = (vals_are_index && shift == begin_bit) ? nullptr : vals;
         hipLaunchKernelGGL(k_rs_scatter<KeyT>, dim3(nblocks), dim3(RS_THREADS), 0, s, keys, vin, keys_alt,
                            vals_alt, n, shift, hist, rowsum, nblocks);
