@@ -609,7 +609,8 @@ __global__ __launch_bounds__(64) void k_point(const uint32_t* __restrict__ slots
 // order (far terms, then the triangles of near leaves), followed by the subtrees of its near
 // children in reverse slot order (they were pushed in slot order and are popped last first).  So
 // two more level-parallel passes over the records compute, bottom up, every record's number of
-// terms T and, top down, where its terms start in that order (a slot's own terms after the
+// terms T (kept in its parent's slot entry, beside the other slots' own term counts, so that a
+// pass reads one word per slot) and, top down, where its terms start in that order (a slot's own terms after the
 // node's earlier own terms; a child's subtree after the node's own terms and the subtrees of the
 // children in later slots), writing each term's pair (a, b) to that position of a list.  One
 // chain of fmaf over the list gives the sum: far terms as fmaf(t, inv3, omega) and triangle
@@ -619,7 +620,7 @@ __global__ __launch_bounds__(64) void k_point(const uint32_t* __restrict__ slots
 // levels writes no result and marks its done byte 2 (the host answers that point with k_point);
 // every other point marks it 1.
 constexpr uint32_t kRecLeaf64 = 0u, kRecFar = 1u, kRecInner = 2u, kRecLeaf32 = 3u;
-constexpr int kBfsLevels = 64;
+constexpr int kBfsLevels = 63; // level bounds live in the lanes of one register (lane L: first record of level L)
 // slot entry: kind in bits 0-1; a near internal slot: child record << 2; an FP32 leaf:
 // (triangles - 1) << 2 | first value << 5
 
@@ -666,18 +667,19 @@ __global__ __launch_bounds__(64) void k_point_bfs(const uint32_t* __restrict__ s
                                                   uint8_t* __restrict__ done)
 {
     // LDS: the list of term pairs (at most one per slot entry and FP32 value); per record its
-    // node link, T and start; per slot entry its kind word (kind 0 for a node's unused slots),
-    // t and inv3; the FP32 triangle values
-    __shared__ int lvl[kBfsLevels + 1]; // level L holds records [lvl[L], lvl[L + 1])
+    // node link, its parent's slot entry and its start; per slot entry its kind word (kind 0 for
+    // a node's unused slots), t, inv3 and its term count (own terms; T of the child for a near
+    // internal slot); the FP32 triangle values
     extern __shared__ float2 terms[];
     const int nsl = max_rec * width; // slot entries
     uint32_t* rec_node = reinterpret_cast<uint32_t*>(terms + nsl + max_tv);
-    int* rec_T = reinterpret_cast<int*>(rec_node + max_rec);
-    int* rec_S = rec_T + max_rec;
+    int* rec_par = reinterpret_cast<int*>(rec_node + max_rec);
+    int* rec_S = rec_par + max_rec;
     uint32_t* rec_k = reinterpret_cast<uint32_t*>(rec_S + max_rec);
     float* rec_a = reinterpret_cast<float*>(rec_k + nsl);
     float* rec_b = rec_a + nsl;
-    float* trival = rec_b + nsl;
+    int* slot_n = reinterpret_cast<int*>(rec_b + nsl);
+    float* trival = reinterpret_cast<float*>(slot_n + nsl);
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     const float qx = Q ? Q[3 * (size_t)i] : pa.q[i][0];
@@ -690,11 +692,12 @@ __global__ __launch_bounds__(64) void k_point_bfs(const uint32_t* __restrict__ s
     float om = 0.0f;
     unsigned long long acc_lane = 0ull; // this lane's FP64 near leaves, 2^-40 fixed point
     bool redo = false;
+    int lvl = 0; // lane L: the first record of level L (level L holds records [lvl_L, lvl_L+1))
     if (root >= 0) {
         if (lane == 0) {
             rec_node[0] = ((uint32_t)root << 4) | (uint32_t)root_ns;
             rec_S[0] = 0;
-            lvl[0] = 0;
+            rec_par[0] = -1;
         }
         __syncthreads();
         // pass 1: the walk, a level at a time
@@ -774,7 +777,10 @@ __global__ __launch_bounds__(64) void k_point_bfs(const uint32_t* __restrict__ s
                     break;
                 }
                 const int child = nfree + __popcll(im & lt_mask);
-                if (inner) rec_node[child] = link;
+                if (inner) {
+                    rec_node[child] = link;
+                    rec_par[child] = r * width + ls;
+                }
                 if (use32) {
                     float* tv = trival + tvfree + pre;
                     tv[0] = tri_f32(tw, qx, qy, qz);
@@ -796,56 +802,60 @@ __global__ __launch_bounds__(64) void k_point_bfs(const uint32_t* __restrict__ s
                                                      : kRecLeaf64));
                     rec_a[k] = t;
                     rec_b[k] = inv3;
+                    slot_n[k] = (act && far) ? 1 : (use32 ? (int)tcnt : 0); // internal: set by pass 2
                 }
                 nfree += cnt;
                 tvfree += tot;
             }
             if (redo) break;
             ++nlev;
-            if (lane == 0) lvl[nlev] = e; // the level just walked is [b, e); the next, [e, nfree)
+            if (lane == nlev) lvl = e; // the level just walked is [b, e); the next, [e, nfree)
             __syncthreads(); // the level's records before the next level reads them
             b = e;
             e = nfree;
         }
         if (!redo) {
-            // pass 2, bottom up: T, the terms of a record's subtree
+            // pass 2, bottom up: T, the terms of a record's subtree, into its parent's slot entry
+            int t_root = 0;
             for (int L = nlev - 1; L >= 0; --L) {
-                const int lb = lvl[L], le = lvl[L + 1];
+                const int lb = __builtin_amdgcn_readlane(lvl, L), le = __builtin_amdgcn_readlane(lvl, L + 1);
                 for (int c = lb; c < le; c += per) {
                     const int r = c + ln;
                     const bool in_r = ln < per && r < le;
-                    int n = 0;
+                    int n = 0, par = -1;
                     if (in_r) {
-                        const uint32_t kk = rec_k[r * width + ls];
-                        const uint32_t kind = kk & 3u;
-                        n = kind == kRecFar ? 1
-                                            : (kind == kRecLeaf32 ? (int)((kk >> 2) & 7u) + 1
-                                                                  : (kind == kRecInner ? rec_T[kk >> 2] : 0));
+                        n = slot_n[r * width + ls];
+                        par = rec_par[r];
                     }
                     n = seg_prefix(n, ls, width) + seg_suffix(n, ls, width) - n; // the group's total
-                    if (in_r && ls == 0) rec_T[r] = n;
+                    if (in_r && ls == 0) {
+                        if (par >= 0) slot_n[par] = n;
+                        else t_root = n;
+                    }
                 }
                 __syncthreads();
             }
+            t_root = __builtin_amdgcn_readfirstlane(t_root); // lane 0 holds the root's record
             // pass 3, top down: where each record's terms start; each term's location in the list
             for (int L = 0; L < nlev; ++L) {
-                const int lb = lvl[L], le = lvl[L + 1];
+                const int lb = __builtin_amdgcn_readlane(lvl, L), le = __builtin_amdgcn_readlane(lvl, L + 1);
                 for (int c = lb; c < le; c += per) {
                     const int r = c + ln;
                     const bool in_r = ln < per && r < le;
                     const int k = r * width + ls;
                     uint32_t kk = 0u;
-                    int sr = 0;
+                    int sr = 0, nk = 0;
                     float ta = 0.0f, tb = 0.0f;
-                    if (in_r) {
+                    if (in_r) { // one round trip: every word of the slot and the record's start
                         kk = rec_k[k];
                         sr = rec_S[r];
                         ta = rec_a[k];
                         tb = rec_b[k];
+                        nk = slot_n[k];
                     }
                     const uint32_t kind = kk & 3u;
-                    const int own = kind == kRecFar ? 1 : (kind == kRecLeaf32 ? (int)((kk >> 2) & 7u) + 1 : 0);
-                    const int sub = kind == kRecInner ? rec_T[kk >> 2] : 0;
+                    const int own = kind == kRecInner ? 0 : nk;
+                    const int sub = kind == kRecInner ? nk : 0;
                     const int own_incl = seg_prefix(own, ls, width);
                     const int own_all = own_incl + seg_suffix(own, ls, width) - own; // the node's own terms
                     const int sub_incl = seg_suffix(sub, ls, width);
@@ -863,7 +873,7 @@ __global__ __launch_bounds__(64) void k_point_bfs(const uint32_t* __restrict__ s
             }
             // the sum, in list order: one fmaf per term (every lane runs the chain on broadcast
             // reads, 16 reads in flight ahead of their fmaf)
-            const int nt = min(rec_T[0], nsl + max_tv); // at most one term per slot entry and value
+            const int nt = min(t_root, nsl + max_tv); // at most one term per slot entry and value
             int j = 0;
             for (; j + 16 <= nt; j += 16) {
                 float2 v[16];
@@ -1168,7 +1178,7 @@ void run_queries_pipelined(Engine& e, const float* Q, int64_t nQ, float* omega, 
 // byte 2) go to one k_point launch.  k_point's LDS node buffers (one per stack position the tree
 // can need) are used when they fit in kPointLds bytes.  These calls record no device time.
 constexpr size_t kPointLds = 32768;
-constexpr size_t kPointBfsLds = 65536 - 512; // k_point_bfs: records, FP32 triangle values, term list (+ level table)
+constexpr size_t kPointBfsLds = 65536; // k_point_bfs: records, FP32 triangle values, term list
 constexpr int kPointBfsValues = 1024;
 
 void run_points(Engine& e, const float* Q, int64_t nQ, float* omega, uint8_t* inside, int banks)
@@ -1218,7 +1228,7 @@ void run_points(Engine& e, const float* Q, int64_t nQ, float* omega, uint8_t* in
     };
     // breadth-first walk, records in LDS
     const int max_tv = kPointBfsValues;
-    const int max_rec = (int)((kPointBfsLds / sizeof(uint32_t) - 3 * (size_t)max_tv) / (size_t)(3 + 5 * e.width));
+    const int max_rec = (int)((kPointBfsLds / sizeof(uint32_t) - 3 * (size_t)max_tv) / (size_t)(3 + 6 * e.width));
     hipLaunchKernelGGL(k_point_bfs, grid, block, kPointBfsLds, s, e.slots.as<uint32_t>(), e.width,
                        e.tris.as<uint32_t>(), root, root_ns, tau, dq, pa, max_rec, max_tv, hom, hin, hdone);
     FWN_LAUNCH_CHECK("k_point_bfs");
