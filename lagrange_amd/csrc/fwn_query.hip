@@ -45,6 +45,14 @@ __device__ __forceinline__ int mesh_of64(const int64_t* off, int M, int64_t t)
     return lo;
 }
 
+// a value equal in every lane of the wave, made visibly uniform (scalar registers)
+__device__ __forceinline__ int64_t wave_uniform64(int64_t v)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 // 32-bit point keys: mesh id in the top mesh_bits bits, then the leading bits of the point's
 // 30-bit Morton code in its mesh's box.  Only the 24 top bits are sorted: a wave needs its 64
 // points close together, not in exact curve order, and every point's result is independent of
@@ -55,7 +63,14 @@ __global__ __launch_bounds__(QT) void k_query_keys(const float* __restrict__ Q, 
 {
     int64_t i = (int64_t)blockIdx.x * QT + threadIdx.x;
     if (i >= nQ) return;
-    int m = (M > 1) ? mesh_of64(q_off, M, i) : 0;
+    int m = 0;
+    if (M > 1) {
+        // the wave's first and last points (wave-uniform: scalar loads); only a wave that spans a
+        // mesh boundary searches per lane
+        const int64_t w0 = wave_uniform64(i - (threadIdx.x & 63));
+        const int m0 = mesh_of64(q_off, M, w0), m1 = mesh_of64(q_off, M, wave_uniform64(min(w0 + 63, nQ - 1)));
+        m = (m0 == m1) ? m0 : mesh_of64(q_off, M, i);
+    }
     const float* mi = minfo + 8 * m;
     uint32_t mc = morton30(Q[3 * i], Q[3 * i + 1], Q[3 * i + 2], mi, mi + 3);
     const int mbits = 32 - mesh_bits; // bits left for the Morton code
