@@ -468,7 +468,9 @@ def bench_point_calls(mesh, n: int = 10000, threads: int = 16, runs: int = 3):
     out["cpu_restated_one_thread"] = {"us_per_point": 1e6 * dt / n, "points_per_s": n / dt,
                                       "num_insides": int(O.inside_from_omega(om).sum())}
     bk = out.get("batched_call_us", {})
-    cross = [int(k) for k, us in bk.items() if us < int(k) * out["serial_us_per_call"]]
+    # a batched call of one point is a per-point call: the crossover is the smallest k >= 2 at
+    # which one batched call of k points beats k per-point calls
+    cross = [int(k) for k, us in bk.items() if int(k) >= 2 and us < int(k) * out["serial_us_per_call"]]
     out["crossover_points"] = min(cross) if cross else None
     out["workload"] = (f"{F.shape[0]} triangles; {n} points from std::mt19937 (default seed), uniform in the "
                        f"float bbox; {threads} threads x {n} calls for the threaded rate")
