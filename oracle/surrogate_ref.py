@@ -1,8 +1,9 @@
 """CPU restatement of the surrogate's forward and backward pass (TEST INFRASTRUCTURE ONLY).
 
 Same network as lagrange_amd/csrc/fwn_surrogate.hip: multi-resolution hash grid (dense levels
-indexed x + (r+1)(y + (r+1) z), hashed levels x ^ 2654435761 y ^ 805459861 z mod T, trilinear
-weights), 32 -> 128 x 4 -> 1 ReLU MLP, mean squared error.  Arithmetic is FP64; with
+indexed x + (r+1)(y + (r+1) z); hashed levels in 4 x 4 x 4 vertex blocks: block b =
+(x/4) ^ 2654435761 (y/4) ^ 805459861 (z/4) mod T/64, entry 64 b + 16 (z%4) + 4 (y%4) + x%4;
+trilinear weights), 32 -> 128 x 4 -> 1 ReLU MLP, mean squared error.  Arithmetic is FP64; with
 ``emulate=True`` the values the engine rounds are rounded the same way (FP16 table, BF16 encoded
 features, BF16 weights and hidden activations, BF16 output-layer data gradient and hidden data
 gradients), so the engine's gradients can be compared tightly.  ``emulate=False`` is the plain
@@ -60,8 +61,13 @@ def corners(p, res, dense, T):
             n = np.uint64(res + 1)
             ix = x + n * (y + n * z)
         else:
-            ix = (x ^ ((y * np.uint64(2654435761)) & np.uint64(0xFFFFFFFF))
-                  ^ ((z * np.uint64(805459861)) & np.uint64(0xFFFFFFFF))) & np.uint64(T - 1)
+            # vertices in 4 x 4 x 4 blocks: the block coordinates hashed, the vertex's place
+            # in its block in the low 6 bits
+            bx, by, bz = x >> np.uint64(2), y >> np.uint64(2), z >> np.uint64(2)
+            b = (bx ^ ((by * np.uint64(2654435761)) & np.uint64(0xFFFFFFFF))
+                 ^ ((bz * np.uint64(805459861)) & np.uint64(0xFFFFFFFF))) & np.uint64((T - 1) >> 6)
+            ix = (b << np.uint64(6)) | ((z & np.uint64(3)) << np.uint64(4)) | ((y & np.uint64(3)) << np.uint64(2)) \
+                | (x & np.uint64(3))
         wx = f[:, 0] if dx else 1 - f[:, 0]
         wy = f[:, 1] if dy else 1 - f[:, 1]
         wz = f[:, 2] if dz else 1 - f[:, 2]
